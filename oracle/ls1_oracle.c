@@ -1,0 +1,1079 @@
+/*
+ * ls1_oracle.c -- CPU restatement (scalar FP64 C99) of ls1-MarDyn's
+ * short-range force path: linked-cell geometry and binning, periodic wrap and
+ * halo copies, the C08 cell-pair traversal, the VectorizedCellProcessor
+ * site-pair bodies with their masks and macroscopic sums, FullMolecule's SoA
+ * set-up and calcFM, and the Leapfrog half steps.
+ *
+ * TEST INFRASTRUCTURE ONLY. It is the checker the tests, the smoke check and
+ * the benchmark's cpu_baseline leg compare the HIP path against; nothing in
+ * the product links or calls it.
+ *
+ * Every function names the reference lines whose behaviour it restates
+ * (paths relative to the ls1-MarDyn source root). The arithmetic follows the
+ * reference's NOVEC double-precision build: IEEE 1/x and sqrt, no fused
+ * multiply-add (built with -ffp-contract=off).
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *  - pair bodies: closed-form KATs (VectorizedCellProcessorTest.cpp:45-134),
+ *    the reference's own potforce.h evaluated through oracle/_ref, and
+ *    finite differences F = -dU/dr, torque = -e x dU/de;
+ *  - traversal / enumeration: brute-force O(N^2) minimum-image sums, the
+ *    reference's Legacy-path output on its Vectorization*.inp fixtures
+ *    (through oracle/_ref), and a CellProcessorStub-style coverage count;
+ *  - cell index: exhaustive brute-force point-in-box search;
+ *  - Leapfrog: FullMolecule::upd_preF/upd_postF through oracle/_ref, plus
+ *    energy conservation and a free-rotor invariant;
+ *  - component mass/inertia and LJ tables: Component.cpp / Comp2Param.cpp
+ *    through oracle/_ref.
+ */
+#include "ls1_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OX_MAXCOMP 16
+#define OX_MAXSITE 8
+
+typedef struct {
+    int nlj, nc, nd, nq;
+    double lj[OX_MAXSITE][3], lj_m[OX_MAXSITE], lj_eps[OX_MAXSITE],
+        lj_sig[OX_MAXSITE], lj_shift6[OX_MAXSITE];
+    double c[OX_MAXSITE][3], c_m[OX_MAXSITE], c_q[OX_MAXSITE];
+    double d[OX_MAXSITE][3], d_e[OX_MAXSITE][3], d_p[OX_MAXSITE];
+    double qd[OX_MAXSITE][3], q_e[OX_MAXSITE][3], q_Q[OX_MAXSITE];
+    double m, I[3], invI[3];
+    int rotdof;
+    int lj_base; /* first LJ lookup id (EnsembleBase.cpp:98-111) */
+} ox_comp;
+
+typedef struct {
+    int ncomp;
+    ox_comp comp[OX_MAXCOMP];
+    int nlj_total;
+    double *eps24, *sig2, *shift6; /* nlj_total x nlj_total */
+    double epsRF, rc, rcLJ, epsRFInvrc3;
+    double bbmin[3], bbmax[3];
+    int periodic;
+    /* LinkedCells geometry */
+    int boxcells[3], cpd[3];
+    double cell_len[3], cell_rlen[3], halo_min[3], halo_max[3];
+} ox_ctx;
+
+/* ------------------------------------------------------------------ model */
+
+/* Component::calculateLJshift, src/molecules/Component.cpp:118-122 */
+static double lj_shift6(double eps, double sigma, double rc) {
+    double sigperrc2 = sigma * sigma / (rc * rc);
+    double sigperrc6 = sigperrc2 * sigperrc2 * sigperrc2;
+    return 24.0 * eps * (sigperrc6 - sigperrc6 * sigperrc6);
+}
+
+/* Component::updateMassInertia(Site&), src/molecules/Component.cpp:151-179:
+ * only LJ centres and charges carry mass; sites are in the principal frame. */
+static void add_mass_site(double *m, double *I6, int *rotdof, const double *r,
+                          double sm) {
+    *m += sm;
+    I6[0] += sm * (r[1] * r[1] + r[2] * r[2]);
+    I6[1] += sm * (r[0] * r[0] + r[2] * r[2]);
+    I6[2] += sm * (r[0] * r[0] + r[1] * r[1]);
+    *rotdof = 3;
+    for (int d = 0; d < 3; ++d)
+        if (I6[d] == 0.) --(*rotdof);
+}
+
+static int parse_components(ox_ctx *x, int ncomp, const double *desc, int len) {
+    int p = 0, lj_base = 0;
+    if (ncomp < 1 || ncomp > OX_MAXCOMP) return -1;
+    x->ncomp = ncomp;
+    for (int ci = 0; ci < ncomp; ++ci) {
+        ox_comp *c = &x->comp[ci];
+        double I6[3] = {0, 0, 0};
+        int rotdof = 3;
+        memset(c, 0, sizeof(*c));
+        if (p + 4 > len) return -2;
+        c->nlj = (int)desc[p];
+        c->nc = (int)desc[p + 1];
+        c->nd = (int)desc[p + 2];
+        c->nq = (int)desc[p + 3];
+        p += 4;
+        if (c->nlj > OX_MAXSITE || c->nc > OX_MAXSITE || c->nd > OX_MAXSITE ||
+            c->nq > OX_MAXSITE)
+            return -3;
+        if (p + 8 * c->nlj + 5 * c->nc + 7 * c->nd + 7 * c->nq + 3 > len) return -2;
+        rotdof = 0;
+        /* ASCIIReader.cpp:171-177 -> Component::addLJcenter (Component.cpp:103-116) */
+        for (int s = 0; s < c->nlj; ++s) {
+            const double *v = desc + p;
+            for (int d = 0; d < 3; ++d) c->lj[s][d] = v[d];
+            c->lj_m[s] = v[3];
+            c->lj_eps[s] = v[4];
+            c->lj_sig[s] = v[5];
+            c->lj_shift6[s] = (v[7] != 0.) ? lj_shift6(v[4], v[5], v[6]) : 0.0;
+            add_mass_site(&c->m, I6, &rotdof, c->lj[s], c->lj_m[s]);
+            p += 8;
+        }
+        for (int s = 0; s < c->nc; ++s) {
+            const double *v = desc + p;
+            for (int d = 0; d < 3; ++d) c->c[s][d] = v[d];
+            c->c_m[s] = v[3];
+            c->c_q[s] = v[4];
+            add_mass_site(&c->m, I6, &rotdof, c->c[s], c->c_m[s]);
+            p += 5;
+        }
+        for (int s = 0; s < c->nd; ++s) {
+            const double *v = desc + p;
+            for (int d = 0; d < 3; ++d) {
+                c->d[s][d] = v[d];
+                c->d_e[s][d] = v[3 + d];
+            }
+            c->d_p[s] = v[6];
+            p += 7;
+        }
+        for (int s = 0; s < c->nq; ++s) {
+            const double *v = desc + p;
+            for (int d = 0; d < 3; ++d) {
+                c->qd[s][d] = v[d];
+                c->q_e[s][d] = v[3 + d];
+            }
+            c->q_Q[s] = v[6];
+            p += 7;
+        }
+        /* ASCIIReader.cpp:197-202: positive header moments override */
+        for (int d = 0; d < 3; ++d) {
+            double Iov = desc[p + d];
+            c->I[d] = (Iov > 0.) ? Iov : I6[d];
+            /* FullMolecule::updateMassInertia, FullMolecule.h:88-99 */
+            c->invI[d] = (c->I[d] != 0.) ? 1. / c->I[d] : 0.;
+        }
+        p += 3;
+        c->rotdof = rotdof;
+        c->lj_base = lj_base;
+        lj_base += c->nlj;
+    }
+    x->nlj_total = lj_base;
+    return p;
+}
+
+/* Comp2Param::initialize LJ part, src/molecules/Comp2Param.cpp:11-110, laid
+ * out as VectorizedCellProcessor's _eps_sig/_shift6 tables
+ * (VectorizedCellProcessor.cpp:40-80): row/column = LJ lookup id. Same
+ * component: site i's own shift6 (Comp2Param.cpp:27-38); different
+ * components: always truncated-shifted at rcLJ (Comp2Param.cpp:79-105). */
+static void build_lj_tables(ox_ctx *x, const double *mix) {
+    int n = x->nlj_total;
+    x->eps24 = (double *)calloc((size_t)n * n + 1, sizeof(double));
+    x->sig2 = (double *)calloc((size_t)n * n + 1, sizeof(double));
+    x->shift6 = (double *)calloc((size_t)n * n + 1, sizeof(double));
+    int mixidx = 0;
+    double xi_tab[OX_MAXCOMP][OX_MAXCOMP], eta_tab[OX_MAXCOMP][OX_MAXCOMP];
+    for (int i = 0; i < x->ncomp; ++i)
+        for (int j = i + 1; j < x->ncomp; ++j) {
+            xi_tab[i][j] = mix ? mix[2 * mixidx] : 1.0;
+            eta_tab[i][j] = mix ? mix[2 * mixidx + 1] : 1.0;
+            ++mixidx;
+        }
+    for (int ci = 0; ci < x->ncomp; ++ci) {
+        const ox_comp *a = &x->comp[ci];
+        for (int si = 0; si < a->nlj; ++si) {
+            for (int sj = 0; sj < a->nlj; ++sj) {
+                double e24 = 24. * sqrt(a->lj_eps[si] * a->lj_eps[sj]);
+                double s2 = .5 * (a->lj_sig[si] + a->lj_sig[sj]);
+                s2 *= s2;
+                int k = (a->lj_base + si) * n + (a->lj_base + sj);
+                x->eps24[k] = e24;
+                x->sig2[k] = s2;
+                x->shift6[k] = a->lj_shift6[si];
+            }
+        }
+        for (int cj = ci + 1; cj < x->ncomp; ++cj) {
+            const ox_comp *b = &x->comp[cj];
+            double xi = xi_tab[ci][cj], eta = eta_tab[ci][cj];
+            for (int pass = 0; pass < 2; ++pass) {
+                const ox_comp *p1 = pass ? b : a, *p2 = pass ? a : b;
+                for (int s1 = 0; s1 < p1->nlj; ++s1)
+                    for (int s2i = 0; s2i < p2->nlj; ++s2i) {
+                        double e24 = 24. * (xi * sqrt(p1->lj_eps[s1] * p2->lj_eps[s2i]));
+                        double s2 = 0.5 * (eta * (p1->lj_sig[s1] + p2->lj_sig[s2i]));
+                        s2 *= s2;
+                        double sr2 = s2 / (x->rcLJ * x->rcLJ);
+                        double sr6 = sr2 * sr2 * sr2;
+                        int k = (p1->lj_base + s1) * n + (p2->lj_base + s2i);
+                        x->eps24[k] = e24;
+                        x->sig2[k] = s2;
+                        x->shift6[k] = e24 * (sr6 - sr6 * sr6);
+                    }
+            }
+        }
+    }
+}
+
+/* LinkedCells::LinkedCells (cell_mode 0, double rc; LinkedCells.cpp:38-106)
+ * and LinkedCells::rebuild (cell_mode 1, float rc; LinkedCells.cpp:132-201). */
+static void build_geometry(ox_ctx *x, int cell_mode) {
+    for (int d = 0; d < 3; ++d) {
+        double diff = x->bbmax[d] - x->bbmin[d];
+        double rcc = cell_mode ? (double)(float)x->rc : x->rc;
+        int nb = (int)floor(diff / rcc);
+        if (nb == 0) nb = 1;
+        x->boxcells[d] = nb;
+        x->cell_len[d] = diff / nb;
+        x->cell_rlen[d] = nb / diff;
+        x->halo_min[d] = x->bbmin[d] - x->cell_len[d];
+        x->halo_max[d] = x->bbmax[d] + x->cell_len[d];
+        x->cpd[d] = nb + 2;
+    }
+}
+
+void *ox_create(int ncomp, const double *desc, int desc_len, const double *mix,
+                double epsRF, double rc, double rcLJ, const double *box,
+                int periodic, int cell_mode) {
+    ox_ctx *x = (ox_ctx *)calloc(1, sizeof(ox_ctx));
+    if (!x) return NULL;
+    if (parse_components(x, ncomp, desc, desc_len) < 0) {
+        free(x);
+        return NULL;
+    }
+    x->epsRF = epsRF;
+    x->rc = rc;
+    x->rcLJ = rcLJ;
+    /* VectorizedCellProcessor ctor, VectorizedCellProcessor.cpp:19-23 */
+    x->epsRFInvrc3 = 2. * (epsRF - 1.) / ((rc * rc * rc) * (2. * epsRF + 1.));
+    for (int d = 0; d < 3; ++d) {
+        x->bbmin[d] = 0.0;
+        x->bbmax[d] = box[d];
+    }
+    x->periodic = periodic;
+    build_lj_tables(x, mix);
+    build_geometry(x, cell_mode);
+    return x;
+}
+
+void ox_destroy(void *h) {
+    ox_ctx *x = (ox_ctx *)h;
+    if (!x) return;
+    free(x->eps24);
+    free(x->sig2);
+    free(x->shift6);
+    free(x);
+}
+
+int ox_nlj_total(void *h) { return ((ox_ctx *)h)->nlj_total; }
+
+void ox_lj_tables(void *h, double *eps24, double *sig2, double *shift6) {
+    ox_ctx *x = (ox_ctx *)h;
+    size_t n = (size_t)x->nlj_total * x->nlj_total;
+    memcpy(eps24, x->eps24, n * sizeof(double));
+    memcpy(sig2, x->sig2, n * sizeof(double));
+    memcpy(shift6, x->shift6, n * sizeof(double));
+}
+
+void ox_component_props(void *h, double *mass, double *inertia, int *rotdof) {
+    ox_ctx *x = (ox_ctx *)h;
+    for (int c = 0; c < x->ncomp; ++c) {
+        mass[c] = x->comp[c].m;
+        for (int d = 0; d < 3; ++d) inertia[3 * c + d] = x->comp[c].I[d];
+        rotdof[c] = x->comp[c].rotdof;
+    }
+}
+
+void ox_cell_geometry(void *h, int *cpd, double *cl, double *hmin, double *hmax) {
+    ox_ctx *x = (ox_ctx *)h;
+    for (int d = 0; d < 3; ++d) {
+        cpd[d] = x->cpd[d];
+        cl[d] = x->cell_len[d];
+        hmin[d] = x->halo_min[d];
+        hmax[d] = x->halo_max[d];
+    }
+}
+
+/* -------------------------------------------------------------- binning */
+
+/* CellBorderAndFlagManager::getBorder, CellBorderAndFlagManager.h:134-152 */
+static double border(const ox_ctx *x, int idx, int d) {
+    if (idx == 0) return x->halo_min[d];
+    if (idx == 1) return x->bbmin[d];
+    if (idx == x->cpd[d] - 1) return x->bbmax[d];
+    if (idx == x->cpd[d]) return x->halo_max[d];
+    return idx * x->cell_len[d] + x->halo_min[d];
+}
+
+static long lin3(const ox_ctx *x, long i, long j, long k) {
+    /* LinkedCells::cellIndexOf3DIndex, LinkedCells.cpp:925-927 */
+    return (k * x->cpd[1] + j) * x->cpd[0] + i;
+}
+
+/* LinkedCells::getCellIndexOfPoint, LinkedCells.cpp:870-923 */
+long ox_cell_index_of_point(void *h, const double *p) {
+    const ox_ctx *x = (const ox_ctx *)h;
+    int ci[3];
+    double lp[3] = {p[0], p[1], p[2]};
+    for (int d = 0; d < 3; ++d) {
+        if (lp[d] <= x->halo_min[d])
+            lp[d] += x->cell_len[d] * 0.5;
+        else if (lp[d] >= x->halo_max[d])
+            lp[d] -= x->cell_len[d] * 0.5;
+        int v = (int)floor((lp[d] - x->bbmin[d]) * x->cell_rlen[d]) + 1;
+        if (v < 0) v = 0;
+        if (v > x->cpd[d] - 1) v = x->cpd[d] - 1;
+        ci[d] = v;
+    }
+    int inside = 1;
+    for (int d = 0; d < 3; ++d)
+        if (!(border(x, ci[d], d) <= lp[d] && lp[d] < border(x, ci[d] + 1, d))) inside = 0;
+    if (!inside) {
+        for (int d = 0; d < 3; ++d) {
+            if (lp[d] < border(x, ci[d], d))
+                ci[d]--;
+            else if (lp[d] >= border(x, ci[d] + 1, d))
+                ci[d]++;
+        }
+    }
+    return lin3(x, ci[0], ci[1], ci[2]);
+}
+
+void ox_cell_index_batch(void *h, int n, const double *r, long *out) {
+    for (int i = 0; i < n; ++i) out[i] = ox_cell_index_of_point(h, r + 3 * i);
+}
+
+static int is_halo_cell(const ox_ctx *x, long c) {
+    long i = c % x->cpd[0], j = (c / x->cpd[0]) % x->cpd[1], k = c / ((long)x->cpd[0] * x->cpd[1]);
+    return i == 0 || j == 0 || k == 0 || i == x->cpd[0] - 1 || j == x->cpd[1] - 1 ||
+           k == x->cpd[2] - 1;
+}
+
+/* C08 offsets and base-cell walk: C08BasedTraversals.h:46-149,
+ * C08CellPairTraversal.h:45-75,196-228. Emits (c1,c2) as the traversal hands
+ * them to the cell processor (halo-halo skipped, non-halo cell first). */
+static const int c08_pairs[14][2][3] = {
+    {{0, 0, 0}, {0, 0, 0}}, {{0, 0, 0}, {0, 1, 0}}, {{0, 1, 0}, {0, 0, 1}},
+    {{0, 0, 0}, {0, 0, 1}}, {{0, 0, 0}, {0, 1, 1}}, {{1, 0, 0}, {0, 1, 1}},
+    {{1, 0, 0}, {0, 1, 0}}, {{1, 0, 0}, {0, 0, 1}}, {{0, 0, 0}, {1, 0, 0}},
+    {{0, 0, 0}, {1, 1, 0}}, {{1, 1, 0}, {0, 0, 1}}, {{0, 1, 0}, {1, 0, 1}},
+    {{0, 0, 0}, {1, 0, 1}}, {{0, 0, 0}, {1, 1, 1}}};
+
+typedef void (*pair_cb)(void *user, long c1, long c2);
+
+static int walk_c08(const ox_ctx *x, pair_cb cb, void *user) {
+    int npairs = 0;
+    for (int col = 0; col < 8; ++col) {
+        int s0 = col & 1, s1 = (col >> 1) & 1, s2 = (col >> 2) & 1;
+        for (int k = s2; k < x->cpd[2] - 1; k += 2)
+            for (int j = s1; j < x->cpd[1] - 1; j += 2)
+                for (int i = s0; i < x->cpd[0] - 1; i += 2) {
+                    for (int p = 0; p < 14; ++p) {
+                        const int *a = c08_pairs[p][0], *b = c08_pairs[p][1];
+                        long c1 = lin3(x, i + a[0], j + a[1], k + a[2]);
+                        long c2 = lin3(x, i + b[0], j + b[1], k + b[2]);
+                        int h1 = is_halo_cell(x, c1), h2 = is_halo_cell(x, c2);
+                        if (h1 && h2) continue;
+                        if (c1 != c2 && h1) {
+                            long t = c1;
+                            c1 = c2;
+                            c2 = t;
+                        }
+                        cb(user, c1, c2);
+                        ++npairs;
+                    }
+                }
+    }
+    return npairs;
+}
+
+typedef struct {
+    int *out;
+    int max, n;
+} pair_sink;
+
+static void sink_pair(void *u, long c1, long c2) {
+    pair_sink *s = (pair_sink *)u;
+    if (s->n < s->max) {
+        s->out[2 * s->n] = (int)c1;
+        s->out[2 * s->n + 1] = (int)c2;
+    }
+    s->n++;
+}
+
+int ox_c08_pairs(void *h, int *out, int max_pairs) {
+    pair_sink s = {out, max_pairs, 0};
+    walk_c08((const ox_ctx *)h, sink_pair, &s);
+    return s.n;
+}
+
+/* DomainDecompBase::handleDomainLeavingParticles, DomainDecompBase.cpp:199-249:
+ * per dimension x,y,z, molecules below the box get +L (clamped below bbmax),
+ * molecules at or above bbmax get -L (clamped to bbmin). */
+void ox_wrap(void *h, int n, double *r) {
+    const ox_ctx *x = (const ox_ctx *)h;
+    for (int d = 0; d < 3; ++d) {
+        double L = x->bbmax[d] - x->bbmin[d];
+        for (int i = 0; i < n; ++i) {
+            double v = r[3 * i + d];
+            if (v < x->bbmin[d]) {
+                v = v + L;
+                if (v >= x->bbmax[d]) v = nextafter(x->bbmax[d], -INFINITY);
+            } else if (v >= x->bbmax[d]) {
+                v = v - L;
+                if (v <= x->bbmin[d]) v = x->bbmin[d];
+            }
+            r[3 * i + d] = v;
+        }
+    }
+}
+
+/* ------------------------------------------------------------ pair bodies */
+
+static inline double dot3(const double *a, const double *b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* Canonical site-pair bodies. Each returns the force on the FIRST site
+ * (the second gets -f), torques on oriented sites, and the energy term.
+ * kinds: 0 LJ, 1 C-C, 2 C-D, 3 D-D, 4 C-Q, 5 D-Q, 6 Q-Q. */
+enum { K_LJ = 0, K_CC, K_CD, K_DD, K_CQ, K_DQ, K_QQ };
+
+typedef struct {
+    double f[3], M1[3], M2[3], u, rf;
+} pair_out;
+
+/* _loopBodyLJ, VectorizedCellProcessor.cpp:171-224 (u = upot6 term) */
+static void body_lj(const double *d, double eps24, double sig2, double shift6,
+                    pair_out *o) {
+    double r2 = dot3(d, d);
+    double r2inv = 1.0 / r2;
+    double lj2 = sig2 * r2inv, lj4 = lj2 * lj2, lj6 = lj4 * lj2, lj12 = lj6 * lj6;
+    double lj12m6 = lj12 - lj6;
+    double scale = (eps24 * r2inv) * (lj12 + lj12m6);
+    for (int k = 0; k < 3; ++k) o->f[k] = d[k] * scale;
+    o->u = eps24 * lj12m6 + shift6;
+}
+
+/* _loopBodyCharge, VectorizedCellProcessor.cpp:227-277 (NOVEC: sqrt of 1/r2) */
+static void body_cc(const double *d, double q1, double q2, pair_out *o) {
+    double r2inv = 1.0 / dot3(d, d);
+    double rinv = sqrt(r2inv);
+    double upot = (q1 * q2) * rinv;
+    double fac = upot * r2inv;
+    for (int k = 0; k < 3; ++k) o->f[k] = d[k] * fac;
+    o->u = upot;
+}
+
+/* _loopBodyChargeDipole, VectorizedCellProcessor.cpp:279-347: first = charge,
+ * second = dipole (axis e, moment p); M2 = torque on the dipole. */
+static void body_cd(const double *d, double q, const double *e, double p,
+                    pair_out *o) {
+    double r2inv = 1.0 / dot3(d, d);
+    double rinv = sqrt(r2inv);
+    double r3inv = r2inv * rinv;
+    double re = dot3(d, e);
+    double qpr3 = (q * p) * r3inv;
+    double fac = r2inv * 3.0 * re;
+    for (int k = 0; k < 3; ++k) o->f[k] = qpr3 * (e[k] - d[k] * fac);
+    o->u = -(qpr3 * re);
+    o->M2[0] = qpr3 * (e[1] * d[2] - e[2] * d[1]);
+    o->M2[1] = qpr3 * (e[2] * d[0] - e[0] * d[2]);
+    o->M2[2] = qpr3 * (e[0] * d[1] - e[1] * d[0]);
+}
+
+/* _loopBodyDipole, VectorizedCellProcessor.cpp:349-442 (with reaction field) */
+static void body_dd(const double *d, const double *e1, double p1, const double *e2,
+                    double p2, double epsRFInvrc3, pair_out *o) {
+    double r2inv = 1.0 / dot3(d, d);
+    double rinv = sqrt(r2inv);
+    double r2three = 3.0 * r2inv;
+    double p1p2 = p1 * p2;
+    double rffac = p1p2 * epsRFInvrc3;
+    double p1p2r3 = p1p2 * rinv * r2inv;
+    double p1p2r5 = p1p2r3 * r2three;
+    double e1e2 = dot3(e1, e2), re1 = dot3(d, e1), re2 = dot3(d, e2);
+    double re1t = re1 * r2three, re2t = re2 * r2three;
+    double re1re2 = r2inv * re1 * re2;
+    double g = e1e2 - 5.0 * re1re2;
+    for (int k = 0; k < 3; ++k) o->f[k] = p1p2r5 * (d[k] * g + e1[k] * re2 + e2[k] * re1);
+    o->u = p1p2r3 * (e1e2 - 3.0 * re1re2);
+    o->rf = rffac * e1e2;
+    double c12[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                     e1[0] * e2[1] - e1[1] * e2[0]};
+    double e1xd[3] = {e1[1] * d[2] - e1[2] * d[1], e1[2] * d[0] - e1[0] * d[2],
+                      e1[0] * d[1] - e1[1] * d[0]};
+    double e2xd[3] = {e2[1] * d[2] - e2[2] * d[1], e2[2] * d[0] - e2[0] * d[2],
+                      e2[0] * d[1] - e2[1] * d[0]};
+    for (int k = 0; k < 3; ++k) {
+        o->M1[k] = p1p2r3 * (re2t * e1xd[k] - c12[k]) + rffac * c12[k];
+        o->M2[k] = p1p2r3 * (re1t * e2xd[k] + c12[k]) - rffac * c12[k];
+    }
+}
+
+/* _loopBodyChargeQuadrupole, VectorizedCellProcessor.cpp:444-524: first =
+ * charge, second = linear quadrupole (axis e, moment m); M2 on the quadrupole. */
+static void body_cq(const double *d, double q, const double *e, double m,
+                    pair_out *o) {
+    double r2inv = 1.0 / dot3(d, d);
+    double rinv = sqrt(r2inv);
+    double qQ05 = 0.5 * q * m;
+    double costj = dot3(e, d) * rinv;
+    double qQr3 = qQ05 * rinv * r2inv;
+    double upot = qQr3 * (3.0 * costj * costj - 1.0);
+    double mpR = 3.0 * upot * r2inv;
+    double pT = 6.0 * costj * qQr3 * rinv;
+    double fac = costj * pT * rinv + mpR;
+    for (int k = 0; k < 3; ++k) o->f[k] = fac * d[k] - pT * e[k];
+    o->u = upot;
+    o->M2[0] = pT * (e[2] * d[1] - e[1] * d[2]);
+    o->M2[1] = pT * (e[0] * d[2] - e[2] * d[0]);
+    o->M2[2] = pT * (e[1] * d[0] - e[0] * d[1]);
+}
+
+/* _loopBodyDipoleQuadrupole, VectorizedCellProcessor.cpp:526-648: first =
+ * dipole (ei, p), second = quadrupole (ej, m). */
+static void body_dq(const double *d, const double *ei, double p, const double *ej,
+                    double m, pair_out *o) {
+    double r2inv = 1.0 / dot3(d, d);
+    double rinv = sqrt(r2inv);
+    double myqfac = 1.5 * p * m * r2inv * r2inv;
+    double costi = dot3(ei, d) * rinv, costj = dot3(ej, d) * rinv;
+    double cos2tj = costj * costj;
+    double cosgij = dot3(ei, ej);
+    double c5 = 5.0 * cos2tj - 1.0;
+    double upot = myqfac * (2.0 * costj * cosgij - costi * c5);
+    double myqfacr = myqfac * rinv;
+    double mpR = 4.0 * upot * r2inv;
+    double mpTi = myqfacr * c5;
+    double part1 = 5.0 * (costi * costj) - cosgij;
+    double mpTj = myqfacr * 2.0 * part1;
+    double pG = myqfac * (2.0 * costj);
+    double part3 = costi * mpTi + costj * mpTj;
+    double fac = mpR - part3 * rinv;
+    for (int k = 0; k < 3; ++k) o->f[k] = fac * d[k] + mpTi * ei[k] + mpTj * ej[k];
+    o->u = upot;
+    double c12[3] = {ei[1] * ej[2] - ei[2] * ej[1], ei[2] * ej[0] - ei[0] * ej[2],
+                     ei[0] * ej[1] - ei[1] * ej[0]};
+    double eixd[3] = {ei[1] * d[2] - ei[2] * d[1], ei[2] * d[0] - ei[0] * d[2],
+                      ei[0] * d[1] - ei[1] * d[0]};
+    double ejxd[3] = {ej[1] * d[2] - ej[2] * d[1], ej[2] * d[0] - ej[0] * d[2],
+                      ej[0] * d[1] - ej[1] * d[0]};
+    for (int k = 0; k < 3; ++k) {
+        o->M1[k] = mpTi * eixd[k] - pG * c12[k];
+        o->M2[k] = mpTj * ejxd[k] + pG * c12[k];
+    }
+}
+
+/* _loopBodyQuadrupole, VectorizedCellProcessor.cpp:650-792 */
+static void body_qq(const double *d, const double *ei, double mi, const double *ej,
+                    double mj, pair_out *o) {
+    double r2inv = 1.0 / dot3(d, d);
+    double rinv = sqrt(r2inv);
+    double qfac = 0.75 * rinv;
+    qfac = qfac * (mi * mj);
+    qfac = qfac * (r2inv * r2inv);
+    double costi = dot3(ei, d) * rinv, costj = dot3(ej, d) * rinv;
+    double c2i = costi * costi, c2j = costj * costj;
+    double cosgij = dot3(ei, ej);
+    double term = cosgij - 5.0 * (costi * costj);
+    double upot = 5.0 * (c2i + c2j) + 15.0 * c2i * c2j;
+    upot = (1.0 + 2.0 * term * term) - upot;
+    upot = qfac * upot;
+    double mpR = 5.0 * upot * r2inv;
+    double part1 = qfac * 10.0 * rinv;
+    double part2 = 2.0 * term;
+    double mpTi = part1 * (costi + (part2 * costj + 3.0 * costi * c2j));
+    double mpTj = part1 * (costj + (part2 * costi + 3.0 * costj * c2i));
+    double pG = qfac * 4.0 * term;
+    double fac = mpR - (mpTi * costi + mpTj * costj) * rinv;
+    for (int k = 0; k < 3; ++k) o->f[k] = fac * d[k] + mpTi * ei[k] + mpTj * ej[k];
+    o->u = upot;
+    double c12[3] = {ei[1] * ej[2] - ei[2] * ej[1], ei[2] * ej[0] - ei[0] * ej[2],
+                     ei[0] * ej[1] - ei[1] * ej[0]};
+    double eixd[3] = {ei[1] * d[2] - ei[2] * d[1], ei[2] * d[0] - ei[0] * d[2],
+                      ei[0] * d[1] - ei[1] * d[0]};
+    double ejxd[3] = {ej[1] * d[2] - ej[2] * d[1], ej[2] * d[0] - ej[0] * d[2],
+                      ej[0] * d[1] - ej[1] * d[0]};
+    for (int k = 0; k < 3; ++k) {
+        o->M1[k] = mpTi * eixd[k] - pG * c12[k];
+        o->M2[k] = mpTj * ejxd[k] + pG * c12[k];
+    }
+}
+
+void ox_site_pair(int kind, const double *ri, const double *ei, double pi,
+                  const double *rj, const double *ej, double pj, double eps24,
+                  double sig2, double shift6, double epsRFInvrc3, double *out) {
+    pair_out o;
+    memset(&o, 0, sizeof(o));
+    double d[3] = {ri[0] - rj[0], ri[1] - rj[1], ri[2] - rj[2]};
+    switch (kind) {
+    case K_LJ: body_lj(d, eps24, sig2, shift6, &o); break;
+    case K_CC: body_cc(d, pi, pj, &o); break;
+    case K_CD: body_cd(d, pi, ej, pj, &o); break;
+    case K_DD: body_dd(d, ei, pi, ej, pj, epsRFInvrc3, &o); break;
+    case K_CQ: body_cq(d, pi, ej, pj, &o); break;
+    case K_DQ: body_dq(d, ei, pi, ej, pj, &o); break;
+    case K_QQ: body_qq(d, ei, pi, ej, pj, &o); break;
+    }
+    for (int k = 0; k < 3; ++k) {
+        out[k] = o.f[k];
+        out[3 + k] = o.M1[k];
+        out[6 + k] = o.M2[k];
+    }
+    out[9] = o.u;
+    out[10] = o.rf;
+}
+
+/* ---------------------------------------------- molecule SoA (per molecule) */
+
+typedef struct {
+    double r[3];           /* molecule position */
+    int cid;
+    int orig;              /* index of the inner molecule this is (a copy of) */
+    double q[4];           /* normalised quaternion */
+    double s[4][OX_MAXSITE][3]; /* site positions per type LJ,C,D,Q */
+    double e[4][OX_MAXSITE][3]; /* orientation of D,Q sites */
+    double f[4][OX_MAXSITE][3], V[4][OX_MAXSITE][3], Ms[4][OX_MAXSITE][3];
+} ox_mol;
+
+/* Quaternion::rotate, src/molecules/Quaternion.cpp:43-64 */
+static void qrot(const double *q, const double *d, double *out) {
+    out[0] = out[1] = out[2] = 0.0;
+    if (d[0] == 0. && d[1] == 0. && d[2] == 0.) return;
+    double ww = q[0] * q[0], xx = q[1] * q[1], yy = q[2] * q[2], zz = q[3] * q[3];
+    double wx = q[0] * q[1], wy = q[0] * q[2], wz = q[0] * q[3];
+    double xy = q[1] * q[2], xz = q[1] * q[3], yz = q[2] * q[3];
+    out[0] = (ww + xx - yy - zz) * d[0] + 2. * (xy - wz) * d[1] + 2. * (wy + xz) * d[2];
+    out[1] = 2. * (wz + xy) * d[0] + (ww - xx + yy - zz) * d[1] + 2. * (yz - wx) * d[2];
+    out[2] = 2. * (xz - wy) * d[0] + 2. * (wx + yz) * d[1] + (ww - xx - yy + zz) * d[2];
+}
+
+/* Quaternion::rotateinv, Quaternion.cpp:66-86 */
+static void qrotinv(const double *q, const double *d, double *out) {
+    out[0] = out[1] = out[2] = 0.0;
+    if (d[0] == 0. && d[1] == 0. && d[2] == 0.) return;
+    double ww = q[0] * q[0], xx = q[1] * q[1], yy = q[2] * q[2], zz = q[3] * q[3];
+    double wx = q[0] * q[1], wy = q[0] * q[2], wz = q[0] * q[3];
+    double xy = q[1] * q[2], xz = q[1] * q[3], yz = q[2] * q[3];
+    out[0] = (ww + xx - yy - zz) * d[0] + 2. * (xy + wz) * d[1] + 2. * (xz - wy) * d[2];
+    out[1] = 2. * (xy - wz) * d[0] + (ww - xx + yy - zz) * d[1] + 2. * (yz + wx) * d[2];
+    out[2] = 2. * (xz + wy) * d[0] + 2. * (yz - wx) * d[1] + (ww - xx - yy + zz) * d[2];
+}
+
+/* Quaternion::normalize (Quaternion.h:40-42): scale by 1/sqrt(|q|^2) */
+static void qnormalize(double *q) {
+    double s = 1. / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int k = 0; k < 4; ++k) q[k] *= s;
+}
+
+/* FullMolecule::setupSoACache, FullMolecule.cpp:711-767, and clearFM :476-521 */
+static void setup_mol(const ox_ctx *x, ox_mol *m, const double *r, const double *q,
+                      int cid, int orig) {
+    const ox_comp *c = &x->comp[cid];
+    memset(m, 0, sizeof(*m));
+    for (int k = 0; k < 3; ++k) m->r[k] = r[k];
+    for (int k = 0; k < 4; ++k) m->q[k] = q[k];
+    qnormalize(m->q);
+    m->cid = cid;
+    m->orig = orig;
+    double t[3];
+    for (int s = 0; s < c->nlj; ++s) {
+        qrot(m->q, c->lj[s], t);
+        for (int k = 0; k < 3; ++k) m->s[0][s][k] = t[k] + r[k];
+    }
+    for (int s = 0; s < c->nc; ++s) {
+        qrot(m->q, c->c[s], t);
+        for (int k = 0; k < 3; ++k) m->s[1][s][k] = t[k] + r[k];
+    }
+    for (int s = 0; s < c->nd; ++s) {
+        qrot(m->q, c->d[s], t);
+        for (int k = 0; k < 3; ++k) m->s[2][s][k] = t[k] + r[k];
+        qrot(m->q, c->d_e[s], m->e[2][s]);
+    }
+    for (int s = 0; s < c->nq; ++s) {
+        qrot(m->q, c->qd[s], t);
+        for (int k = 0; k < 3; ++k) m->s[3][s][k] = t[k] + r[k];
+        qrot(m->q, c->q_e[s], m->e[3][s]);
+    }
+}
+
+/* One site pair of a molecule pair: apply force with Newton 3, virial on both
+ * sites, torques to oriented sites and (if macro) the macroscopic sums.
+ * first/second are given in the canonical body order. */
+static void apply_pair(const ox_ctx *x, ox_mol *A, int ta, int sa, ox_mol *B, int tb,
+                       int sb, int kind, double eps24, double sig2, double shift6,
+                       int macro, double *mac, long long *cnt) {
+    pair_out o;
+    memset(&o, 0, sizeof(o));
+    const double *ra = A->s[ta][sa], *rb = B->s[tb][sb];
+    double d[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+    const ox_comp *ca = &x->comp[A->cid], *cb = &x->comp[B->cid];
+    switch (kind) {
+    case K_LJ: body_lj(d, eps24, sig2, shift6, &o); break;
+    case K_CC: body_cc(d, ca->c_q[sa], cb->c_q[sb], &o); break;
+    case K_CD: body_cd(d, ca->c_q[sa], B->e[2][sb], cb->d_p[sb], &o); break;
+    case K_DD:
+        body_dd(d, A->e[2][sa], ca->d_p[sa], B->e[2][sb], cb->d_p[sb], x->epsRFInvrc3, &o);
+        break;
+    case K_CQ: body_cq(d, ca->c_q[sa], B->e[3][sb], cb->q_Q[sb], &o); break;
+    case K_DQ: body_dq(d, A->e[2][sa], ca->d_p[sa], B->e[3][sb], cb->q_Q[sb], &o); break;
+    case K_QQ: body_qq(d, A->e[3][sa], ca->q_Q[sa], B->e[3][sb], cb->q_Q[sb], &o); break;
+    }
+    double V[3];
+    for (int k = 0; k < 3; ++k) {
+        V[k] = (A->r[k] - B->r[k]) * o.f[k];
+        A->f[ta][sa][k] += o.f[k];
+        B->f[tb][sb][k] -= o.f[k];
+        A->V[ta][sa][k] += V[k];
+        B->V[tb][sb][k] += V[k];
+        A->Ms[ta][sa][k] += o.M1[k];
+        B->Ms[tb][sb][k] += o.M2[k];
+    }
+    if (cnt) cnt[1 + kind] += 1;
+    if (macro) {
+        if (kind == K_LJ)
+            mac[OX_UPOT6LJ] += o.u;
+        else
+            mac[OX_UPOTXPOLES] += o.u;
+        mac[OX_VIRIAL] += V[0] + V[1] + V[2];
+        if (kind == K_DD) mac[OX_MYRF] -= o.rf;
+    }
+}
+
+/* One molecule pair. The cutoff is tested on molecule centres
+ * (calcDistLookup, SIMD_VectorizedCellProcessorHelpers.h:427-472): LJ sites
+ * against rcLJ^2, electrostatic sites against rc^2, strictly less. */
+static void mol_pair(const ox_ctx *x, ox_mol *A, ox_mol *B, int macro, double *mac,
+                     long long *cnt) {
+    double dm[3] = {A->r[0] - B->r[0], A->r[1] - B->r[1], A->r[2] - B->r[2]};
+    double r2 = dm[0] * dm[0] + dm[1] * dm[1] + dm[2] * dm[2];
+    if (cnt) cnt[OX_CNT_MOLPAIRS] += 1;
+    const ox_comp *ca = &x->comp[A->cid], *cb = &x->comp[B->cid];
+    if (r2 < x->rcLJ * x->rcLJ) {
+        int n = x->nlj_total;
+        for (int i = 0; i < ca->nlj; ++i)
+            for (int j = 0; j < cb->nlj; ++j) {
+                int k = (ca->lj_base + i) * n + (cb->lj_base + j);
+                apply_pair(x, A, 0, i, B, 0, j, K_LJ, x->eps24[k], x->sig2[k], x->shift6[k],
+                           macro, mac, cnt);
+            }
+    }
+    if (!(r2 < x->rc * x->rc)) return;
+    /* charge, dipole and quadrupole loops of _calculatePairs
+     * (VectorizedCellProcessor.cpp:1181-2720); every ordered type pair once,
+     * in canonical body order with roles swapped where the reference swaps */
+    for (int i = 0; i < ca->nc; ++i) {
+        for (int j = 0; j < cb->nc; ++j) apply_pair(x, A, 1, i, B, 1, j, K_CC, 0, 0, 0, macro, mac, cnt);
+        for (int j = 0; j < cb->nd; ++j) apply_pair(x, A, 1, i, B, 2, j, K_CD, 0, 0, 0, macro, mac, cnt);
+        for (int j = 0; j < cb->nq; ++j) apply_pair(x, A, 1, i, B, 3, j, K_CQ, 0, 0, 0, macro, mac, cnt);
+    }
+    for (int i = 0; i < ca->nd; ++i) {
+        for (int j = 0; j < cb->nc; ++j) apply_pair(x, B, 1, j, A, 2, i, K_CD, 0, 0, 0, macro, mac, cnt);
+        for (int j = 0; j < cb->nd; ++j) apply_pair(x, A, 2, i, B, 2, j, K_DD, 0, 0, 0, macro, mac, cnt);
+        for (int j = 0; j < cb->nq; ++j) apply_pair(x, A, 2, i, B, 3, j, K_DQ, 0, 0, 0, macro, mac, cnt);
+    }
+    for (int i = 0; i < ca->nq; ++i) {
+        for (int j = 0; j < cb->nc; ++j) apply_pair(x, B, 1, j, A, 3, i, K_CQ, 0, 0, 0, macro, mac, cnt);
+        for (int j = 0; j < cb->nd; ++j) apply_pair(x, B, 2, j, A, 3, i, K_DQ, 0, 0, 0, macro, mac, cnt);
+        for (int j = 0; j < cb->nq; ++j) apply_pair(x, A, 3, i, B, 3, j, K_QQ, 0, 0, 0, macro, mac, cnt);
+    }
+}
+
+/* FullMolecule::calcFM, FullMolecule.cpp:548-631 (site offsets from
+ * computeXX_d = rotate(q, body offset), FullMolecule.h:213-239) */
+static void calc_fm(const ox_ctx *x, const ox_mol *m, double *F, double *M, double *Vi) {
+    const ox_comp *c = &x->comp[m->cid];
+    double tM[3] = {0, 0, 0}, tV[3] = {0, 0, 0};
+    F[0] = F[1] = F[2] = 0;
+    M[0] = M[1] = M[2] = 0;
+    int ns[4] = {c->nlj, c->nc, c->nd, c->nq};
+    for (int t = 0; t < 4; ++t) {
+        for (int s = 0; s < ns[t]; ++s) {
+            const double *body = t == 0 ? c->lj[s] : t == 1 ? c->c[s] : t == 2 ? c->d[s] : c->qd[s];
+            double ds[3];
+            qrot(m->q, body, ds);
+            const double *fs = m->f[t][s];
+            for (int k = 0; k < 3; ++k) F[k] += fs[k];
+            M[0] += ds[1] * fs[2] - ds[2] * fs[1];
+            M[1] += ds[2] * fs[0] - ds[0] * fs[2];
+            M[2] += ds[0] * fs[1] - ds[1] * fs[0];
+            for (int k = 0; k < 3; ++k) tV[k] += m->V[t][s][k];
+            if (t >= 2)
+                for (int k = 0; k < 3; ++k) tM[k] += m->Ms[t][s][k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        Vi[k] = 0.5 * tV[k];
+        M[k] += tM[k];
+    }
+}
+
+/* ---------------------------------------------------------------- traversal */
+
+typedef struct {
+    ox_ctx *x;
+    ox_mol *mols;
+    int *cell_start, *cell_list;
+    double *mac;
+    long long *cnt;
+} trav_state;
+
+/* VectorizedCellProcessor::processCell / processCellPair,
+ * VectorizedCellProcessor.cpp:2732-2819 (sumAll = false path) */
+static void trav_cb(void *u, long c1, long c2) {
+    trav_state *t = (trav_state *)u;
+    int b1 = t->cell_start[c1], e1 = t->cell_start[c1 + 1];
+    if (c1 == c2) {
+        if (is_halo_cell(t->x, c1) || e1 - b1 < 2) return;
+        for (int a = b1; a < e1; ++a)
+            for (int b = a + 1; b < e1; ++b) {
+                ox_mol *A = &t->mols[t->cell_list[a]], *B = &t->mols[t->cell_list[b]];
+                double dm[3] = {A->r[0] - B->r[0], A->r[1] - B->r[1], A->r[2] - B->r[2]};
+                /* SingleCellPolicy_::GetForceMask: m_r2 != 0 */
+                if (dm[0] * dm[0] + dm[1] * dm[1] + dm[2] * dm[2] == 0.0) {
+                    if (t->cnt) t->cnt[OX_CNT_MOLPAIRS] += 1;
+                    continue;
+                }
+                mol_pair(t->x, A, B, 1, t->mac, t->cnt);
+            }
+        return;
+    }
+    int b2 = t->cell_start[c2], e2 = t->cell_start[c2 + 1];
+    if (e1 == b1 || e2 == b2) return;
+    int h1 = is_halo_cell(t->x, c1), h2 = is_halo_cell(t->x, c2);
+    if (h1 && h2) return;
+    int macro = (!h1 && !h2) || (c1 < c2);
+    for (int a = b1; a < e1; ++a)
+        for (int b = b2; b < e2; ++b)
+            mol_pair(t->x, &t->mols[t->cell_list[a]], &t->mols[t->cell_list[b]], macro, t->mac,
+                     t->cnt);
+}
+
+/* DomainDecompBase::populateHaloLayerWithCopies, DomainDecompBase.cpp:317-376:
+ * for x, then y, then z, copy every molecule (inner and earlier copies)
+ * within rc of a face to the opposite side. */
+static int make_halo(const ox_ctx *x, ox_mol **pm, int n, int *cap) {
+    int total = n;
+    for (int d = 0; d < 3; ++d) {
+        double L = x->bbmax[d] - x->bbmin[d];
+        for (int dir = 0; dir < 2; ++dir) {
+            double shift = dir == 0 ? L : -L;
+            double lo[3], hi[3];
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = x->bbmin[k] - x->rc;
+                hi[k] = x->bbmax[k] + x->rc;
+            }
+            if (dir == 0) {
+                lo[d] = x->bbmin[d];
+                hi[d] = x->bbmin[d] + x->rc;
+            } else {
+                lo[d] = x->bbmax[d] - x->rc;
+                hi[d] = x->bbmax[d];
+            }
+            int cur = total;
+            for (int i = 0; i < cur; ++i) {
+                ox_mol *m = &(*pm)[i];
+                int in = 1;
+                for (int k = 0; k < 3; ++k)
+                    if (!(lo[k] <= m->r[k] && m->r[k] < hi[k])) in = 0;
+                if (!in) continue;
+                if (total >= *cap) {
+                    *cap *= 2;
+                    *pm = (ox_mol *)realloc(*pm, sizeof(ox_mol) * (size_t)(*cap));
+                    m = &(*pm)[i];
+                }
+                double r[3] = {m->r[0], m->r[1], m->r[2]};
+                r[d] = r[d] + shift;
+                if (shift < 0) {
+                    if (r[d] >= x->bbmin[d]) r[d] = nextafter(x->bbmin[d], -INFINITY);
+                } else {
+                    if (r[d] < x->bbmax[d]) r[d] = nextafter(x->bbmax[d], INFINITY);
+                }
+                ox_mol tmp;
+                setup_mol(x, &tmp, r, m->q, m->cid, m->orig);
+                (*pm)[total++] = tmp;
+            }
+        }
+    }
+    return total;
+}
+
+int ox_forces(void *h, int n, const double *r, double *q, const int *cid, double *F,
+              double *M, double *Vi, double *macro, long long *counters) {
+    ox_ctx *x = (ox_ctx *)h;
+    int cap = n + 16 + n / 2;
+    ox_mol *mols = (ox_mol *)malloc(sizeof(ox_mol) * (size_t)cap);
+    if (!mols) return -1;
+    for (int i = 0; i < n; ++i) {
+        setup_mol(x, &mols[i], r + 3 * i, q + 4 * i, cid[i], i);
+        for (int k = 0; k < 4; ++k) q[4 * i + k] = mols[i].q[k];
+    }
+    int total = n;
+    if (x->periodic) total = make_halo(x, &mols, n, &cap);
+    long ncell = (long)x->cpd[0] * x->cpd[1] * x->cpd[2];
+    int *cs = (int *)calloc((size_t)ncell + 1, sizeof(int));
+    int *cl = (int *)malloc(sizeof(int) * (size_t)(total + 1));
+    long *ci = (long *)malloc(sizeof(long) * (size_t)(total + 1));
+    for (int i = 0; i < total; ++i) {
+        ci[i] = ox_cell_index_of_point(x, mols[i].r);
+        cs[ci[i] + 1]++;
+    }
+    for (long c = 0; c < ncell; ++c) cs[c + 1] += cs[c];
+    int *fill = (int *)calloc((size_t)ncell, sizeof(int));
+    for (int i = 0; i < total; ++i) cl[cs[ci[i]] + fill[ci[i]]++] = i;
+    double mac[4] = {0, 0, 0, 0};
+    long long cnt[OX_NCNT];
+    memset(cnt, 0, sizeof(cnt));
+    trav_state ts = {x, mols, cs, cl, mac, cnt};
+    walk_c08(x, trav_cb, &ts);
+    for (int i = 0; i < n; ++i) calc_fm(x, &mols[i], F + 3 * i, M + 3 * i, Vi + 3 * i);
+    for (int k = 0; k < 4; ++k) macro[k] = mac[k];
+    if (counters)
+        for (int k = 0; k < OX_NCNT; ++k) counters[k] = cnt[k];
+    free(fill);
+    free(ci);
+    free(cl);
+    free(cs);
+    free(mols);
+    return 0;
+}
+
+/* Independent check of the enumeration: every unordered molecule pair once,
+ * minimum image per dimension (requires box > 2 rc in periodic dimensions). */
+int ox_forces_brute(void *h, int n, const double *r, double *q, const int *cid, double *F,
+                    double *M, double *Vi, double *macro) {
+    ox_ctx *x = (ox_ctx *)h;
+    ox_mol *mols = (ox_mol *)malloc(sizeof(ox_mol) * (size_t)(n + 1));
+    ox_mol img;
+    double mac[4] = {0, 0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        setup_mol(x, &mols[i], r + 3 * i, q + 4 * i, cid[i], i);
+        for (int k = 0; k < 4; ++k) q[4 * i + k] = mols[i].q[k];
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            double s[3] = {0, 0, 0};
+            if (x->periodic)
+                for (int d = 0; d < 3; ++d) {
+                    double L = x->bbmax[d] - x->bbmin[d];
+                    double dd = mols[i].r[d] - mols[j].r[d];
+                    if (dd > 0.5 * L) s[d] = L;
+                    else if (dd < -0.5 * L) s[d] = -L;
+                }
+            if (s[0] == 0 && s[1] == 0 && s[2] == 0) {
+                mol_pair(x, &mols[i], &mols[j], 1, mac, NULL);
+            } else {
+                double rr[3] = {mols[j].r[0] + s[0], mols[j].r[1] + s[1], mols[j].r[2] + s[2]};
+                setup_mol(x, &img, rr, mols[j].q, mols[j].cid, j);
+                mol_pair(x, &mols[i], &img, 1, mac, NULL);
+                for (int t = 0; t < 4; ++t)
+                    for (int si = 0; si < OX_MAXSITE; ++si)
+                        for (int k = 0; k < 3; ++k) {
+                            mols[j].f[t][si][k] += img.f[t][si][k];
+                            mols[j].V[t][si][k] += img.V[t][si][k];
+                            mols[j].Ms[t][si][k] += img.Ms[t][si][k];
+                        }
+            }
+        }
+    for (int i = 0; i < n; ++i) calc_fm(x, &mols[i], F + 3 * i, M + 3 * i, Vi + 3 * i);
+    for (int k = 0; k < 4; ++k) macro[k] = mac[k];
+    free(mols);
+    return 0;
+}
+
+/* ----------------------------------------------------------------- Leapfrog */
+
+/* Quaternion::differentiate, Quaternion.cpp:98-103 */
+static void qdiff(const double *q, const double *w, double *dq) {
+    dq[0] = .5 * (-q[1] * w[0] - q[2] * w[1] - q[3] * w[2]);
+    dq[1] = .5 * (q[0] * w[0] - q[3] * w[1] + q[2] * w[2]);
+    dq[2] = .5 * (q[3] * w[0] + q[0] * w[1] - q[1] * w[2]);
+    dq[3] = .5 * (-q[2] * w[0] + q[1] * w[1] + q[0] * w[2]);
+}
+
+/* FullMolecule::upd_preF, FullMolecule.cpp:331-361 (Leapfrog::transition1to2,
+ * Leapfrog.cpp:45-61) */
+void ox_leapfrog_pre(void *h, int n, double dt, double *r, double *v, double *q, double *L,
+                     const double *F, const double *M, const int *cid) {
+    ox_ctx *x = (ox_ctx *)h;
+    for (int i = 0; i < n; ++i) {
+        const ox_comp *c = &x->comp[cid[i]];
+        double dt_halve = .5 * dt;
+        double dtInv2m = dt_halve / c->m;
+        for (int d = 0; d < 3; ++d) {
+            v[3 * i + d] += dtInv2m * F[3 * i + d];
+            r[3 * i + d] += dt * v[3 * i + d];
+        }
+        double *qi = q + 4 * i, *Li = L + 3 * i;
+        double w[3], qh[4], qi2[4];
+        qrotinv(qi, Li, w);
+        for (int d = 0; d < 3; ++d) w[d] *= c->invI[d];
+        qdiff(qi, w, qh);
+        for (int k = 0; k < 4; ++k) qh[k] *= dt_halve;
+        for (int k = 0; k < 4; ++k) qh[k] += qi[k];
+        double qcorr = 1. / sqrt(qh[0] * qh[0] + qh[1] * qh[1] + qh[2] * qh[2] + qh[3] * qh[3]);
+        for (int k = 0; k < 4; ++k) qh[k] *= qcorr;
+        for (int d = 0; d < 3; ++d) Li[d] += dt_halve * M[3 * i + d];
+        qrotinv(qh, Li, w);
+        for (int d = 0; d < 3; ++d) w[d] *= c->invI[d];
+        qdiff(qh, w, qi2);
+        for (int k = 0; k < 4; ++k) qi2[k] *= dt;
+        for (int k = 0; k < 4; ++k) qi[k] += qi2[k];
+        qcorr = 1. / sqrt(qi[0] * qi[0] + qi[1] * qi[1] + qi[2] * qi[2] + qi[3] * qi[3]);
+        for (int k = 0; k < 4; ++k) qi[k] *= qcorr;
+    }
+}
+
+/* FullMolecule::upd_postF, FullMolecule.cpp:363-387 (Leapfrog::transition2to3,
+ * Leapfrog.cpp:63-147, single thermostat): sums[0] = sum m v^2,
+ * sums[1] = sum I w^2, sums[2] = N, sums[3] = rotational DOF. */
+void ox_leapfrog_post(void *h, int n, double dt, double *v, double *L, const double *q,
+                      const double *F, const double *M, const int *cid, double *sums) {
+    ox_ctx *x = (ox_ctx *)h;
+    double summv2 = 0, sumIw2 = 0, rot = 0;
+    double dt_halve = 0.5 * dt;
+    for (int i = 0; i < n; ++i) {
+        const ox_comp *c = &x->comp[cid[i]];
+        double dtInv2m = dt_halve / c->m;
+        double v2 = 0.;
+        for (int d = 0; d < 3; ++d) {
+            v[3 * i + d] += dtInv2m * F[3 * i + d];
+            v2 += v[3 * i + d] * v[3 * i + d];
+            L[3 * i + d] += dt_halve * M[3 * i + d];
+        }
+        summv2 += c->m * v2;
+        double w[3], Iw2 = 0.;
+        qrotinv(q + 4 * i, L + 3 * i, w);
+        for (int d = 0; d < 3; ++d) {
+            w[d] *= c->invI[d];
+            Iw2 += c->I[d] * w[d] * w[d];
+        }
+        sumIw2 += Iw2;
+        rot += c->rotdof;
+    }
+    sums[0] = summv2;
+    sums[1] = sumIw2;
+    sums[2] = n;
+    sums[3] = rot;
+}
+
+/* Simulation::simulateOneTimestep step order (Simulation.cpp:1076-1307)
+ * without thermostat, long-range correction or plugins: upd_preF, periodic
+ * wrap, re-bin + halo + SoA, traversal, calcFM, upd_postF. F/M must hold the
+ * forces of the initial configuration on entry (Simulation.cpp:909-925).
+ * per_step[s*6 + ...] = Upot, virial, summv2, sumIw2, upot6lj/6, upotXpoles+myRF */
+int ox_simulate(void *h, int n, int nsteps, double dt, double *r, double *v, double *q,
+                double *L, const int *cid, double *F, double *M, double *Vi, double *per_step) {
+    double mac[4], sums[4];
+    for (int s = 0; s < nsteps; ++s) {
+        ox_leapfrog_pre(h, n, dt, r, v, q, L, F, M, cid);
+        if (((ox_ctx *)h)->periodic) ox_wrap(h, n, r);
+        if (ox_forces(h, n, r, q, cid, F, M, Vi, mac, NULL)) return -1;
+        ox_leapfrog_post(h, n, dt, v, L, q, F, M, cid, sums);
+        if (per_step) {
+            double *o = per_step + 6 * s;
+            o[0] = mac[OX_UPOT6LJ] / 6.0 + mac[OX_UPOTXPOLES] + mac[OX_MYRF];
+            o[1] = mac[OX_VIRIAL] + 3.0 * mac[OX_MYRF];
+            o[2] = sums[0];
+            o[3] = sums[1];
+            o[4] = mac[OX_UPOT6LJ] / 6.0;
+            o[5] = mac[OX_UPOTXPOLES] + mac[OX_MYRF];
+        }
+    }
+    return 0;
+}

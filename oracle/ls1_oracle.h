@@ -1,0 +1,73 @@
+/*
+ * ls1_oracle.h -- CPU restatement (scalar FP64 C) of ls1-MarDyn's short-range
+ * force path. TEST INFRASTRUCTURE ONLY: the tests, the smoke check and the
+ * benchmark's cpu_baseline leg call it as the checker. The HIP product never
+ * links, loads or calls anything in this directory.
+ *
+ * Component records mirror the ls1 ASCII phase-space header
+ * (src/io/ASCIIReader.cpp:152-232), flattened to doubles, per component:
+ *   nlj nc nd nq
+ *   nlj x (x y z m eps sigma rc_shift do_shift)
+ *   nc  x (x y z m q)
+ *   nd  x (x y z ex ey ez absMy)
+ *   nq  x (x y z ex ey ez absQ)
+ *   I11 I22 I33            (values > 0 override the computed moments)
+ * Mixing coefficients: (xi, eta) for every component pair i<j in the order
+ * (0,1),(0,2),..,(1,2),.. exactly as ASCIIReader.cpp:213-232 reads them.
+ */
+#ifndef LS1_ORACLE_H
+#define LS1_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* macro[] layout used by every force routine */
+enum { OX_UPOT6LJ = 0, OX_UPOTXPOLES = 1, OX_VIRIAL = 2, OX_MYRF = 3 };
+
+/* counters[] layout: molecule pairs whose centre distance was tested,
+ * site pairs inside the cutoff per kernel type */
+enum { OX_CNT_MOLPAIRS = 0, OX_CNT_LJ = 1, OX_CNT_CC = 2, OX_CNT_CD = 3,
+       OX_CNT_DD = 4, OX_CNT_CQ = 5, OX_CNT_DQ = 6, OX_CNT_QQ = 7, OX_NCNT = 8 };
+
+void *ox_create(int ncomp, const double *desc, int desc_len, const double *mix,
+                double epsRF, double rc, double rcLJ, const double *box,
+                int periodic, int cell_mode);
+void ox_destroy(void *h);
+
+int ox_nlj_total(void *h);
+void ox_lj_tables(void *h, double *eps24, double *sig2, double *shift6);
+void ox_component_props(void *h, double *mass, double *inertia, int *rotdof);
+void ox_cell_geometry(void *h, int *cells_per_dim, double *cell_length,
+                      double *halo_min, double *halo_max);
+
+long ox_cell_index_of_point(void *h, const double *p);
+void ox_cell_index_batch(void *h, int n, const double *r, long *out);
+int ox_c08_pairs(void *h, int *out, int max_pairs);
+
+void ox_wrap(void *h, int n, double *r);
+int ox_forces(void *h, int n, const double *r, double *q, const int *cid,
+              double *F, double *M, double *Vi, double *macro,
+              long long *counters);
+int ox_forces_brute(void *h, int n, const double *r, double *q, const int *cid,
+                    double *F, double *M, double *Vi, double *macro);
+void ox_leapfrog_pre(void *h, int n, double dt, double *r, double *v,
+                     double *q, double *L, const double *F, const double *M,
+                     const int *cid);
+void ox_leapfrog_post(void *h, int n, double dt, double *v, double *L,
+                      const double *q, const double *F, const double *M,
+                      const int *cid, double *sums);
+int ox_simulate(void *h, int n, int nsteps, double dt, double *r, double *v,
+                double *q, double *L, const int *cid, double *F, double *M,
+                double *Vi, double *per_step);
+
+/* single site pair, canonical argument order; out = f[3], M1[3], M2[3], u, rf */
+void ox_site_pair(int kind, const double *ri, const double *ei, double pi,
+                  const double *rj, const double *ej, double pj,
+                  double eps24, double sig2, double shift6, double epsRFInvrc3,
+                  double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
