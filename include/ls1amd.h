@@ -1,0 +1,179 @@
+/*
+ * ls1amd.h -- C ABI of the MI355X short-range force path for ls1-MarDyn.
+ *
+ * One context owns one HIP stream on one device and the device-resident
+ * molecule state. Calls on one context must be serialised by the caller (the
+ * reference's CellProcessor is driven from one traversal at a time). Every
+ * call returns MDX_OK (0) or a negative status; mdx_last_error() gives the
+ * message. No exceptions cross the ABI. Arrays are caller-owned host memory,
+ * borrowed for the duration of the call; the mdx_*_device entry points take
+ * device pointers instead.
+ *
+ * Component records use the ls1 ASCII phase-space header layout
+ * (src/io/ASCIIReader.cpp:152-232), flattened to doubles, per component:
+ *   nlj nc nd nq
+ *   nlj x (x y z m eps sigma rc_shift do_shift)
+ *   nc  x (x y z m q)
+ *   nd  x (x y z ex ey ez absMy)
+ *   nq  x (x y z ex ey ez absQ)
+ *   I11 I22 I33        (values > 0 override the moments computed from sites)
+ * Mixing coefficients are (xi, eta) per component pair i<j, in the order
+ * (0,1),(0,2),..,(1,2),.. (ASCIIReader.cpp:213-232).
+ *
+ * Reference interfaces replaced (paths relative to the ls1-MarDyn source):
+ *   CellProcessor            src/particleContainer/adapter/CellProcessor.h:29-94
+ *   VectorizedCellProcessor  src/particleContainer/adapter/VectorizedCellProcessor.h:29-321
+ *   ParticleContainer        src/particleContainer/ParticleContainer.h:85-279
+ *   LinkedCells              src/particleContainer/LinkedCells.cpp
+ *   Leapfrog                 src/integrators/Leapfrog.cpp:32-156
+ */
+#ifndef LS1AMD_H
+#define LS1AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDX_OK 0
+#define MDX_ERR_ARG (-1)      /* bad argument / unsupported configuration */
+#define MDX_ERR_HIP (-2)      /* HIP runtime error */
+#define MDX_ERR_STATE (-3)    /* call out of order (e.g. no model yet) */
+#define MDX_ERR_CAPACITY (-4) /* an internal capacity would be exceeded */
+#define MDX_ERR_NODEVICE (-5) /* no HIP device */
+
+/* precision forms */
+#define MDX_PREC_FP64 0       /* all arithmetic FP64 (reference PRECISION=DOUBLE) */
+#define MDX_PREC_FP32_FP64ACC 1 /* FP32 pair arithmetic, FP64 accumulation,
+                                   FP64 sums and FP64 Leapfrog (PRECISION=MIXED) */
+
+/* macro[] layout: the four sums VectorizedCellProcessor::endTraversal reduces
+ * (VectorizedCellProcessor.cpp:122-155). Upot = macro[0]/6 + macro[1] +
+ * macro[3]; Virial = macro[2] + 3 * macro[3]. */
+#define MDX_UPOT6LJ 0
+#define MDX_UPOTXPOLES 1
+#define MDX_VIRIAL 2
+#define MDX_MYRF 3
+
+typedef struct mdx_context mdx_context;
+
+/* Thread-local message of the last failed call on ctx (or of mdx_create). */
+const char *mdx_last_error(const mdx_context *ctx);
+
+/* Creates a context on HIP device `device` (its own stream, default priority).
+ * Replaces the construction of VectorizedCellProcessor + LinkedCells
+ * (Simulation.cpp:471-519, 851-862). */
+int mdx_create(int device, mdx_context **out);
+int mdx_destroy(mdx_context *ctx);
+
+/* Component model and parameter tables: Component::addLJcenter etc.
+ * (Component.cpp:103-216), Comp2Param::initialize (Comp2Param.cpp:11-202) and
+ * the VCP table build (VectorizedCellProcessor.cpp:19-96). eps_rf feeds
+ * epsRFInvrc3 (VectorizedCellProcessor.cpp:21). mix may be NULL (all 1). */
+int mdx_set_components(mdx_context *ctx, int ncomp, const double *desc, int desc_len,
+                       const double *mix, double eps_rf);
+
+/* Replaces the LJ tables with caller tables (24 eps, sigma^2, 6 * shift) over
+ * all LJ centres, row/column = LJ lookup id, as VectorizedCellProcessor's
+ * _eps_sig/_shift6 (VectorizedCellProcessor.cpp:56-80). n_ljc must equal the
+ * model's LJ centre count. */
+int mdx_set_lj_tables(mdx_context *ctx, int n_ljc, const double *eps24, const double *sig2,
+                      const double *shift6);
+
+/* Domain [0,box) per dimension, cutoffs and cell geometry.
+ * CellProcessor(rc, rcLJ) (CellProcessor.h:37), LinkedCells(bBox, rc)
+ * (LinkedCells.cpp:38-106, cell_mode 0, double rc) or LinkedCells::rebuild
+ * (LinkedCells.cpp:132-201, cell_mode 1, float rc). periodic != 0: periodic
+ * boundaries in all dimensions (DomainDecompBase.cpp:80-376); 0: open
+ * boundaries as in the reference's ParticleContainerFactory test set-up. */
+int mdx_set_domain(mdx_context *ctx, const double *box, double rc, double rc_lj, int periodic,
+                   int cell_mode);
+
+/* MDX_PREC_FP64 (default) or MDX_PREC_FP32_FP64ACC. */
+int mdx_set_precision(mdx_context *ctx, int precision);
+
+/* Molecule state in ls1's ICRVQD order: positions r[3n], velocities v[3n],
+ * quaternions q[4n] (qw qx qy qz), angular momenta L[3n] (lab frame, the
+ * reference's D), component ids cid[n] (0-based). ids may be NULL.
+ * Replaces ParticleContainer::addParticles (ParticleContainer.h:121). */
+int mdx_upload(mdx_context *ctx, int64_t n, const double *r, const double *v, const double *q,
+               const double *L, const int32_t *cid, const uint64_t *ids);
+
+/* Copies the state back in upload order. Any pointer may be NULL. F, M and Vi
+ * are the molecule force, torque and virial of the last traversal
+ * (FullMolecule::calcFM, FullMolecule.cpp:548-631). */
+int mdx_download(mdx_context *ctx, int64_t n, double *r, double *v, double *q, double *L,
+                 double *F, double *M, double *Vi, uint64_t *ids);
+
+/* Periodic wrap + counting-sort re-bin + SoA site build.
+ * Replaces LinkedCells::update (LinkedCells.cpp:241-354),
+ * DomainDecompBase::exchangeMolecules (DomainDecompBase.cpp:80-114) and
+ * LinkedCells::updateMoleculeCaches / FullMolecule::setupSoACache
+ * (LinkedCells.cpp:1097-1107, FullMolecule.cpp:711-767). */
+int mdx_update_container(mdx_context *ctx);
+
+/* Force traversal: every site pair of every molecule pair closer than the
+ * cutoff, then FullMolecule::calcFM. Replaces LinkedCells::traverseCells with
+ * VectorizedCellProcessor (LinkedCells.cpp:565-575: initTraversal,
+ * processCell/processCellPair over the C08 enumeration, endTraversal) and
+ * Simulation::updateForces (Simulation.cpp:835-845). macro may be NULL. */
+int mdx_traverse(mdx_context *ctx, double *macro);
+
+/* Leapfrog::transition1to2 -> FullMolecule::upd_preF (Leapfrog.cpp:45-61,
+ * FullMolecule.cpp:331-361). */
+int mdx_leapfrog_pre(mdx_context *ctx, double dt);
+
+/* Leapfrog::transition2to3 -> FullMolecule::upd_postF (Leapfrog.cpp:63-147,
+ * FullMolecule.cpp:363-387). sums (may be NULL) = {sum m v^2, sum I w^2, N,
+ * rotational DOF}. */
+int mdx_leapfrog_post(mdx_context *ctx, double dt, double *sums);
+
+/* nsteps of Simulation::simulateOneTimestep's force path (Simulation.cpp:
+ * 1076-1307 without thermostat / long-range correction / plugins):
+ * upd_preF, wrap + re-bin, traversal, calcFM, upd_postF. The context must hold
+ * the forces of the current state (call mdx_update_container + mdx_traverse
+ * once first, as Simulation::prepare_start does). per_step (may be NULL) gets
+ * 6 doubles per step: Upot, virial, sum m v^2, sum I w^2, Upot_LJ, Upot_rest.
+ * use_graph != 0 replays one captured hipGraph per step. */
+int mdx_step(mdx_context *ctx, int nsteps, double dt, double *per_step, int use_graph);
+
+/* Blocks until the context's stream is idle. */
+int mdx_synchronize(mdx_context *ctx);
+
+/* ls1 cell index (LinkedCells::getCellIndexOfPoint, LinkedCells.cpp:870-923,
+ * halo layer included) of every molecule, in upload order. */
+int mdx_cell_ids(mdx_context *ctx, int64_t n, int64_t *out);
+
+/* Cell geometry: cells per dimension including the halo layer, cell length. */
+int mdx_cell_geometry(mdx_context *ctx, int32_t *cells_per_dim, double *cell_length);
+
+/* The cell pairs the traversal evaluates, as ls1 would hand them to
+ * processCell / processCellPair (C08CellPairTraversal.h:45-75,
+ * C08BasedTraversals.h:46-149), expressed on ls1's halo-extended cell grid:
+ * pairs[2k], pairs[2k+1]. Returns the count through *count; writes at most
+ * max_pairs. */
+int mdx_cell_pairs(mdx_context *ctx, int32_t *pairs, int64_t max_pairs, int64_t *count);
+
+/* Work counters of the last traversal (FlopCounter categories,
+ * FlopCounter.cpp:19-76): [0] molecule-centre distance tests (each unordered
+ * pair once), [1..7] site pairs inside the cutoff: LJ, C-C, C-D, D-D, C-Q,
+ * D-Q, Q-Q. Counting is enabled by mdx_set_counting(ctx, 1). */
+int mdx_set_counting(mdx_context *ctx, int enable);
+int mdx_counters(mdx_context *ctx, int64_t *out);
+
+/* Device-resident variants for callers that keep the state in HBM: device
+ * pointers to n doubles / ints laid out as in mdx_upload. */
+int mdx_upload_device(mdx_context *ctx, int64_t n, const double *r, const double *v,
+                      const double *q, const double *L, const int32_t *cid);
+
+/* Time of the last traversal's pair kernels in milliseconds, measured with
+ * HIP events on the context's stream (0 if not measured). */
+int mdx_set_timing(mdx_context *ctx, int enable);
+int mdx_last_kernel_ms(mdx_context *ctx, double *pair_ms, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

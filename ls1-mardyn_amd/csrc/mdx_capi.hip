@@ -1,0 +1,1004 @@
+// mdx_capi.hip -- the C ABI of include/ls1amd.h: context, device buffers,
+// model set-up (Component / Comp2Param restated) and the launch sequence.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ls1amd.h"
+#include "mdx_internal.h"
+
+namespace mdx {
+__global__ void k_leapfrog_pre(MolArrays, const double *, const double *, const Component *, Geom,
+                               double, int);
+__global__ void k_leapfrog_post(MolArrays, const double *, const double *, const Component *,
+                                double, int, double *);
+__global__ void k_reduce_partials(const double *, int, int, double *);
+__global__ void k_fill_int(int *, int, int);
+__global__ void k_bin_count(MolArrays, Geom, int, int *, int *, int *, int *);
+__global__ void k_bin_scatter(int, const int *, const int *, const int *, int *);
+__global__ void k_bin_gather(int, MolArrays, MolArrays, const int *, const int *, const int *,
+                             int *, const Component *, int *, int *, int *, int *);
+__global__ void k_scan_block(const int *, int *, int, int *);
+__global__ void k_scan_add(int *, int, const int *);
+__global__ void k_scan_total(const int *, const int *, int, int *);
+__global__ void k_cell_site_start(int, const int *, const int *, int *);
+__global__ void k_build_sites(int, MolArrays, Geom, const int *, const Component *, SiteArrays,
+                              SiteArrays, SiteArrays, SiteArrays);
+__global__ void k_calc_fm(int, MolArrays, const Component *, SiteArrays, SiteArrays, SiteArrays,
+                          SiteArrays, double *, double *, double *);
+__global__ void k_scatter_orig(int, const int *, const double *, double *, int, int);
+__global__ void k_finalize_macro(const double *, double *, double *, const int *);
+__global__ void k_finalize_lf(const double *, double *, const int *);
+__global__ void k_step_advance(int *);
+__global__ void k_enum_pairs(Geom, int, int *);
+}  // namespace mdx
+
+using namespace mdx;
+
+namespace {
+thread_local std::string g_create_error;
+constexpr int kBlock = 256;
+inline int nblk(long n, int b = kBlock) { return (int)std::max<long>(1, (n + b - 1) / b); }
+}  // namespace
+
+struct mdx_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool have_model = false, have_domain = false, have_state = false;
+    int precision = MDX_PREC_FP64;
+    bool counting = false, timing = false;
+    // model
+    Model model{};
+    Component *d_comps = nullptr;
+    std::vector<double> site_eps, site_sig, site_shift6, mix;
+    bool has_mix = false;
+    std::vector<double> eps24, sig2, shift6;
+    double *d_eps24 = nullptr, *d_sig2 = nullptr, *d_shift6 = nullptr;
+    float *d_feps24 = nullptr, *d_fsig2 = nullptr, *d_fshift6 = nullptr;
+    double eps_rf = 1.0;
+    // domain
+    Geom g{};
+    double rc = 0, rc_lj = 0;
+    int ncell = 0;
+    // state
+    int n = 0;
+    int nsites[kTypes] = {0, 0, 0, 0};
+    MolArrays mol[2]{};
+    int cur = 0;
+    double *F = nullptr, *M = nullptr, *Vi = nullptr;
+    int *cell = nullptr, *rank = nullptr, *count = nullptr, *start = nullptr, *perm = nullptr,
+        *mcell = nullptr, *ls1id = nullptr;
+    int *nsite_mol[kTypes] = {nullptr, nullptr, nullptr, nullptr};
+    SiteArrays sites[kTypes]{};
+    std::vector<int *> scan_tmp;
+    double *macro_part = nullptr, *lf_part = nullptr, *red = nullptr, *step_out = nullptr;
+    int *step_idx = nullptr;
+    int cap_steps = 0;
+    unsigned long long *d_counters = nullptr;
+    std::vector<uint64_t> ids;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double pair_ms = 0, total_ms = 0;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    double graph_dt = 0;
+    std::vector<void *> allocs;        // lifetime of the context
+    std::vector<void *> state_allocs;  // re-made by each upload
+};
+
+namespace {
+
+int fail(mdx_context *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                      \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(ctx, MDX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define TRY(expr)                      \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != MDX_OK) return rc_; \
+    } while (0)
+
+template <typename T> int dalloc(mdx_context *c, T **p, size_t count, bool state = false) {
+    void *q = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, MDX_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    (state ? c->state_allocs : c->allocs).push_back(q);
+    *p = (T *)q;
+    return MDX_OK;
+}
+
+void free_list(std::vector<void *> &v) {
+    for (void *p : v) (void)hipFree(p);
+    v.clear();
+}
+
+void drop_graphs(mdx_context *c) {
+    for (auto &g : c->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// Component::calculateLJshift (Component.cpp:118-122)
+double lj_shift6(double eps, double sigma, double rc) {
+    double s2 = sigma * sigma / (rc * rc);
+    double s6 = s2 * s2 * s2;
+    return 24.0 * eps * (s6 - s6 * s6);
+}
+
+// Parses the ls1 ASCII-header component records. Mass and moments follow
+// Component::updateMassInertia (Component.cpp:151-179: only LJ centres and
+// charges carry mass) and FullMolecule::updateMassInertia (FullMolecule.h:
+// 88-99: invI = 0 for a zero moment); positive header moments override
+// (ASCIIReader.cpp:197-202).
+int parse_model(mdx_context *c, int ncomp, const double *desc, int len) {
+    if (ncomp < 1 || ncomp > kMaxComp) return fail(c, MDX_ERR_ARG, "ncomp out of range [1,16]");
+    Model &m = c->model;
+    memset(&m, 0, sizeof(m));
+    m.ncomp = ncomp;
+    c->site_eps.clear();
+    c->site_sig.clear();
+    c->site_shift6.clear();
+    int p = 0, base = 0;
+    for (int ci = 0; ci < ncomp; ++ci) {
+        Component &cp = m.comp[ci];
+        if (p + 4 > len) return fail(c, MDX_ERR_ARG, "component records truncated");
+        for (int t = 0; t < kTypes; ++t) {
+            cp.n[t] = (int)desc[p + t];
+            if (cp.n[t] < 0 || cp.n[t] > kMaxSite)
+                return fail(c, MDX_ERR_ARG, "site count out of range [0,8]");
+        }
+        p += 4;
+        if (p + 8 * cp.n[0] + 5 * cp.n[1] + 7 * cp.n[2] + 7 * cp.n[3] + 3 > len)
+            return fail(c, MDX_ERR_ARG, "component records truncated");
+        double I6[3] = {0, 0, 0};
+        int rotdof = 0;
+        auto add_mass = [&](const double *r, double sm) {
+            cp.m += sm;
+            I6[0] += sm * (r[1] * r[1] + r[2] * r[2]);
+            I6[1] += sm * (r[0] * r[0] + r[2] * r[2]);
+            I6[2] += sm * (r[0] * r[0] + r[1] * r[1]);
+            rotdof = 3;
+            for (int d = 0; d < 3; ++d)
+                if (I6[d] == 0.) --rotdof;
+        };
+        for (int s = 0; s < cp.n[0]; ++s, p += 8) {
+            for (int d = 0; d < 3; ++d) cp.body[0][s][d] = desc[p + d];
+            c->site_eps.push_back(desc[p + 4]);
+            c->site_sig.push_back(desc[p + 5]);
+            c->site_shift6.push_back(desc[p + 7] != 0. ? lj_shift6(desc[p + 4], desc[p + 5],
+                                                                   desc[p + 6])
+                                                       : 0.);
+            add_mass(desc + p, desc[p + 3]);
+        }
+        for (int s = 0; s < cp.n[1]; ++s, p += 5) {
+            for (int d = 0; d < 3; ++d) cp.body[1][s][d] = desc[p + d];
+            cp.param[1][s] = desc[p + 4];
+            add_mass(desc + p, desc[p + 3]);
+        }
+        for (int t = 2; t < 4; ++t)
+            for (int s = 0; s < cp.n[t]; ++s, p += 7) {
+                for (int d = 0; d < 3; ++d) {
+                    cp.body[t][s][d] = desc[p + d];
+                    cp.e[t][s][d] = desc[p + 3 + d];
+                }
+                cp.param[t][s] = desc[p + 6];
+            }
+        for (int d = 0; d < 3; ++d) {
+            cp.I[d] = desc[p + d] > 0. ? desc[p + d] : I6[d];
+            cp.invI[d] = cp.I[d] != 0. ? 1. / cp.I[d] : 0.;
+        }
+        p += 3;
+        cp.rotdof = rotdof;
+        cp.lj_base = base;
+        base += cp.n[0];
+        if (!(cp.m > 0.)) return fail(c, MDX_ERR_ARG, "component has no mass");
+    }
+    m.nlj = base;
+    return MDX_OK;
+}
+
+// Comp2Param::initialize, LJ part (Comp2Param.cpp:11-110), in the layout of
+// VectorizedCellProcessor's _eps_sig / _shift6 (VectorizedCellProcessor.cpp:
+// 40-80): same component -> site i's own shift6; different components ->
+// Lorentz-Berthelot with (xi, eta), always shifted at rcLJ.
+void build_tables(mdx_context *c) {
+    const Model &m = c->model;
+    const int n = m.nlj;
+    const std::vector<double> &eps = c->site_eps, &sig = c->site_sig;
+    c->eps24.assign((size_t)n * n, 0.);
+    c->sig2.assign((size_t)n * n, 0.);
+    c->shift6.assign((size_t)n * n, 0.);
+    int k = 0;
+    std::vector<double> xi(kMaxComp * kMaxComp, 1.0), eta(kMaxComp * kMaxComp, 1.0);
+    for (int i = 0; i < m.ncomp; ++i)
+        for (int j = i + 1; j < m.ncomp; ++j, ++k)
+            if (c->has_mix) {
+                xi[i * kMaxComp + j] = c->mix[2 * k];
+                eta[i * kMaxComp + j] = c->mix[2 * k + 1];
+            }
+    for (int ci = 0; ci < m.ncomp; ++ci) {
+        const Component &a = m.comp[ci];
+        for (int si = 0; si < a.n[0]; ++si)
+            for (int sj = 0; sj < a.n[0]; ++sj) {
+                int ii = a.lj_base + si, jj = a.lj_base + sj;
+                double s2 = .5 * (sig[ii] + sig[jj]);
+                s2 *= s2;
+                c->eps24[(size_t)ii * n + jj] = 24. * sqrt(eps[ii] * eps[jj]);
+                c->sig2[(size_t)ii * n + jj] = s2;
+                c->shift6[(size_t)ii * n + jj] = c->site_shift6[ii];
+            }
+        for (int cj = ci + 1; cj < m.ncomp; ++cj) {
+            const Component &b = m.comp[cj];
+            double x = xi[ci * kMaxComp + cj], h = eta[ci * kMaxComp + cj];
+            for (int pass = 0; pass < 2; ++pass) {
+                const Component &p1 = pass ? b : a, &p2 = pass ? a : b;
+                for (int s1 = 0; s1 < p1.n[0]; ++s1)
+                    for (int s2i = 0; s2i < p2.n[0]; ++s2i) {
+                        int ii = p1.lj_base + s1, jj = p2.lj_base + s2i;
+                        double e24 = 24. * (x * sqrt(eps[ii] * eps[jj]));
+                        double s2 = 0.5 * (h * (sig[ii] + sig[jj]));
+                        s2 *= s2;
+                        double sr2 = s2 / (c->rc_lj * c->rc_lj);
+                        double sr6 = sr2 * sr2 * sr2;
+                        c->eps24[(size_t)ii * n + jj] = e24;
+                        c->sig2[(size_t)ii * n + jj] = s2;
+                        c->shift6[(size_t)ii * n + jj] = e24 * (sr6 - sr6 * sr6);
+                    }
+            }
+        }
+    }
+}
+
+int upload_tables(mdx_context *c) {
+    size_t nn = c->eps24.size();
+    if (nn == 0) return MDX_OK;
+    std::vector<float> fe(nn), fs(nn), fh(nn);
+    for (size_t i = 0; i < nn; ++i) {
+        fe[i] = (float)c->eps24[i];
+        fs[i] = (float)c->sig2[i];
+        fh[i] = (float)c->shift6[i];
+    }
+    HIPCHK(c, hipMemcpy(c->d_eps24, c->eps24.data(), nn * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_sig2, c->sig2.data(), nn * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_shift6, c->shift6.data(), nn * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_feps24, fe.data(), nn * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_fsig2, fs.data(), nn * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_fshift6, fh.data(), nn * 4, hipMemcpyHostToDevice));
+    return MDX_OK;
+}
+
+// LinkedCells ctor (cell_mode 0, LinkedCells.cpp:38-106) or rebuild
+// (cell_mode 1, float cell edge, LinkedCells.cpp:132-201).
+int build_geom(mdx_context *c, const double *box, double rc, int periodic, int cell_mode) {
+    Geom &g = c->g;
+    memset(&g, 0, sizeof(g));
+    long ncell = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (!(box[d] > 0)) return fail(c, MDX_ERR_ARG, "box lengths must be positive");
+        g.bbmin[d] = 0.0;
+        g.bbmax[d] = box[d];
+        g.L[d] = box[d];
+        double rcc = cell_mode ? (double)(float)rc : rc;
+        int nb = (int)floor(box[d] / rcc);
+        if (nb == 0) nb = 1;
+        if (nb < 2)
+            return fail(c, MDX_ERR_ARG,
+                        "bounding box too small for the cell length (LinkedCells needs at "
+                        "least 2 cells per dimension, LinkedCells.cpp:79-96)");
+        g.n[d] = nb;
+        g.cpd[d] = nb + 2;
+        g.cl[d] = box[d] / nb;
+        g.rcl[d] = nb / box[d];
+        g.halo_min[d] = g.bbmin[d] - g.cl[d];
+        g.halo_max[d] = g.bbmax[d] + g.cl[d];
+        ncell *= nb;
+    }
+    if (ncell > (1L << 28)) return fail(c, MDX_ERR_CAPACITY, "too many cells");
+    g.periodic = periodic ? 1 : 0;
+    c->ncell = (int)ncell;
+    return MDX_OK;
+}
+
+void set_cutoff_terms(mdx_context *c) {
+    c->g.rc2 = c->rc * c->rc;
+    c->g.rclj2 = c->rc_lj * c->rc_lj;
+    // VectorizedCellProcessor ctor, VectorizedCellProcessor.cpp:21
+    c->g.epsRFInvrc3 = 2. * (c->eps_rf - 1.) / ((c->rc * c->rc * c->rc) * (2. * c->eps_rf + 1.));
+}
+
+int alloc_mol(mdx_context *c, MolArrays &m, int n) {
+    double **dp[] = {&m.rx, &m.ry, &m.rz, &m.vx, &m.vy, &m.vz, &m.qw,
+                     &m.qx, &m.qy, &m.qz, &m.Lx, &m.Ly, &m.Lz};
+    for (double **p : dp) TRY(dalloc(c, p, n, true));
+    TRY(dalloc(c, &m.cid, n, true));
+    TRY(dalloc(c, &m.orig, n, true));
+    return MDX_OK;
+}
+
+int alloc_sites(mdx_context *c, SiteArrays &s, int cap, int ncell, int n, int type) {
+    memset(&s, 0, sizeof(s));
+    TRY(dalloc(c, &s.cstart, ncell + 1, true));
+    TRY(dalloc(c, &s.mstart, n + 1, true));
+    if (cap == 0) return MDX_OK;
+    TRY(dalloc(c, &s.mol, cap, true));
+    double **dp[] = {&s.x, &s.y, &s.z, &s.mx, &s.my, &s.mz,
+                     &s.Fx, &s.Fy, &s.Fz, &s.Vx, &s.Vy, &s.Vz};
+    for (double **p : dp) TRY(dalloc(c, p, cap, true));
+    float **fp[] = {&s.fx, &s.fy, &s.fz, &s.fmx, &s.fmy, &s.fmz};
+    for (float **p : fp) TRY(dalloc(c, p, cap, true));
+    if (type == T_LJ) {
+        TRY(dalloc(c, &s.ljid, cap, true));
+    } else {
+        TRY(dalloc(c, &s.p, cap, true));
+        TRY(dalloc(c, &s.fp, cap, true));
+    }
+    if (type >= T_D) {
+        double **ep[] = {&s.ex, &s.ey, &s.ez, &s.Mx, &s.My, &s.Mz};
+        for (double **p : ep) TRY(dalloc(c, p, cap, true));
+        float **fe[] = {&s.fex, &s.fey, &s.fez};
+        for (float **p : fe) TRY(dalloc(c, p, cap, true));
+    }
+    return MDX_OK;
+}
+
+int alloc_scan(mdx_context *c, int n) {
+    c->scan_tmp.clear();
+    int len = std::max(n, 1);
+    while (true) {
+        int nb = (len + 1023) / 1024;
+        int *p;
+        TRY(dalloc(c, &p, (size_t)((nb + 1023) / 1024) * 1024 * 2 + 2048, true));
+        c->scan_tmp.push_back(p);
+        if (nb <= 1) break;
+        len = nb;
+    }
+    return MDX_OK;
+}
+
+// Exclusive scan of n ints, out[n] = total, on the context stream.
+int scan_exclusive(mdx_context *c, const int *in, int *out, int n, int level = 0) {
+    int nb = std::max(1, (n + 1023) / 1024);
+    if (level >= (int)c->scan_tmp.size()) return fail(c, MDX_ERR_CAPACITY, "scan depth");
+    int *bsum = c->scan_tmp[level];
+    hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, c->stream, in, out, n, bsum);
+    if (nb > 1) {
+        int *bscan = bsum + ((nb + 1023) / 1024) * 1024;
+        TRY(scan_exclusive(c, bsum, bscan, nb, level + 1));
+        hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, c->stream, out, n, bscan);
+    }
+    if (level == 0)
+        hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, c->stream, in, out, n, out + n);
+    return MDX_OK;
+}
+
+int pair_grid(const mdx_context *c, int *nblocks, int *per_xcd) {
+    *nblocks = (c->ncell + kWavesPerBlock - 1) / kWavesPerBlock;
+    *per_xcd = (*nblocks + 7) / 8;
+    return 8 * *per_xcd;
+}
+
+int n_present(const mdx_context *c) {
+    int k = 0;
+    for (int t = 0; t < kTypes; ++t) k += c->nsites[t] > 0;
+    return k;
+}
+
+// Re-bin (LinkedCells::update + exchangeMolecules' periodic wrap, which
+// leapfrog_pre already applied) and SoA cache build (setupSoACache).
+int launch_update(mdx_context *c) {
+    const int n = c->n;
+    MolArrays &src = c->mol[c->cur], &dst = c->mol[1 - c->cur];
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(c->ncell + 1)), dim3(kBlock), 0, c->stream, c->count,
+                       c->ncell + 1, 0);
+    hipLaunchKernelGGL(k_bin_count, dim3(nblk(n)), dim3(kBlock), 0, c->stream, src, c->g, n,
+                       c->cell, c->rank, c->count, c->ls1id);
+    TRY(scan_exclusive(c, c->count, c->start, c->ncell));
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, c->cell,
+                       c->rank, c->start, c->perm);
+    hipLaunchKernelGGL(k_bin_gather, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, src, dst,
+                       c->cell, c->perm, c->start, c->mcell, c->d_comps, c->nsite_mol[0],
+                       c->nsite_mol[1], c->nsite_mol[2], c->nsite_mol[3]);
+    c->cur = 1 - c->cur;
+    for (int t = 0; t < kTypes; ++t) {
+        if (c->nsites[t] == 0) continue;
+        TRY(scan_exclusive(c, c->nsite_mol[t], c->sites[t].mstart, n));
+        hipLaunchKernelGGL(k_cell_site_start, dim3(nblk(c->ncell + 1)), dim3(kBlock), 0, c->stream,
+                           c->ncell, c->start, c->sites[t].mstart, c->sites[t].cstart);
+    }
+    hipLaunchKernelGGL(k_build_sites, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, c->mol[c->cur],
+                       c->g, c->mcell, c->d_comps, c->sites[0], c->sites[1], c->sites[2],
+                       c->sites[3]);
+    HIPCHK(c, hipGetLastError());
+    return MDX_OK;
+}
+
+int launch_traverse(mdx_context *c) {
+    PairArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g = c->g;
+    for (int t = 0; t < kTypes; ++t) {
+        a.s[t] = c->sites[t];
+        a.present[t] = c->nsites[t] > 0;
+    }
+    a.pp.eps24 = c->d_eps24;
+    a.pp.sig2 = c->d_sig2;
+    a.pp.shift6 = c->d_shift6;
+    a.pp.feps24 = c->d_feps24;
+    a.pp.fsig2 = c->d_fsig2;
+    a.pp.fshift6 = c->d_fshift6;
+    a.pp.nlj = c->model.nlj;
+    a.ncell = c->ncell;
+    int grid = pair_grid(c, &a.nblocks, &a.nblk_per_xcd);
+    a.counters = c->d_counters;
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    int slot = 0;
+    for (int t = 0; t < kTypes; ++t) {
+        if (c->nsites[t] == 0) continue;
+        a.macro_part = c->macro_part + (size_t)slot * grid * 4;
+        PairKernel k = pair_kernel(t, c->precision == MDX_PREC_FP32_FP64ACC, c->counting);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(64 * kWavesPerBlock), 0, c->stream, a);
+        ++slot;
+    }
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->stream, c->macro_part,
+                       slot * grid, 4, c->red);
+    hipLaunchKernelGGL(k_calc_fm, dim3(nblk(c->n)), dim3(kBlock), 0, c->stream, c->n,
+                       c->mol[c->cur], c->d_comps, c->sites[0], c->sites[1], c->sites[2],
+                       c->sites[3], c->F, c->M, c->Vi);
+    hipLaunchKernelGGL(k_finalize_macro, dim3(1), dim3(1), 0, c->stream, c->red, c->red + 4,
+                       c->step_out, c->step_idx);
+    HIPCHK(c, hipGetLastError());
+    return MDX_OK;
+}
+
+int launch_lf_pre(mdx_context *c, double dt) {
+    hipLaunchKernelGGL(k_leapfrog_pre, dim3(nblk(c->n)), dim3(kBlock), 0, c->stream,
+                       c->mol[c->cur], c->F, c->M, c->d_comps, c->g, dt, c->n);
+    HIPCHK(c, hipGetLastError());
+    return MDX_OK;
+}
+
+int launch_lf_post(mdx_context *c, double dt) {
+    int nb = nblk(c->n);
+    hipLaunchKernelGGL(k_leapfrog_post, dim3(nb), dim3(kBlock), 0, c->stream, c->mol[c->cur], c->F,
+                       c->M, c->d_comps, dt, c->n, c->lf_part);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->stream, c->lf_part, nb, 4,
+                       c->red + 8);
+    hipLaunchKernelGGL(k_finalize_lf, dim3(1), dim3(1), 0, c->stream, c->red + 8, c->step_out,
+                       c->step_idx);
+    HIPCHK(c, hipGetLastError());
+    return MDX_OK;
+}
+
+int launch_step(mdx_context *c, double dt) {
+    TRY(launch_lf_pre(c, dt));
+    TRY(launch_update(c));
+    TRY(launch_traverse(c));
+    TRY(launch_lf_post(c, dt));
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, c->stream, c->step_idx);
+    HIPCHK(c, hipGetLastError());
+    return MDX_OK;
+}
+
+int need(mdx_context *c, bool model, bool domain, bool state) {
+    if (model && !c->have_model) return fail(c, MDX_ERR_STATE, "no components set");
+    if (domain && !c->have_domain) return fail(c, MDX_ERR_STATE, "no domain set");
+    if (state && !c->have_state) return fail(c, MDX_ERR_STATE, "no molecules uploaded");
+    return MDX_OK;
+}
+
+int ensure_steps(mdx_context *c, int nsteps) {
+    if (nsteps + 1 <= c->cap_steps) return MDX_OK;
+    int cap = std::max(nsteps + 1, 2 * c->cap_steps);
+    double *p;
+    TRY(dalloc(c, &p, (size_t)6 * cap));
+    c->step_out = p;  // the old buffer stays owned by the context until destroy
+    c->cap_steps = cap;
+    drop_graphs(c);
+    return MDX_OK;
+}
+
+int state_setup(mdx_context *c, int64_t n64, const int32_t *cid_host) {
+    if (n64 <= 0 || n64 > (1 << 27)) return fail(c, MDX_ERR_ARG, "molecule count out of range");
+    const int n = (int)n64;
+    drop_graphs(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_list(c->state_allocs);
+    c->n = n;
+    for (int t = 0; t < kTypes; ++t) c->nsites[t] = 0;
+    for (int i = 0; i < n; ++i) {
+        int k = cid_host[i];
+        if (k < 0 || k >= c->model.ncomp) return fail(c, MDX_ERR_ARG, "component id out of range");
+        for (int t = 0; t < kTypes; ++t) c->nsites[t] += c->model.comp[k].n[t];
+    }
+    for (int t = 0; t < kTypes; ++t)
+        if (c->nsites[t] >= (1 << 27)) return fail(c, MDX_ERR_CAPACITY, "too many sites");
+    for (auto &m : c->mol) TRY(alloc_mol(c, m, n));
+    TRY(dalloc(c, &c->F, 3 * (size_t)n, true));
+    TRY(dalloc(c, &c->M, 3 * (size_t)n, true));
+    TRY(dalloc(c, &c->Vi, 3 * (size_t)n, true));
+    int *ip[] = {nullptr};
+    (void)ip;
+    TRY(dalloc(c, &c->cell, n, true));
+    TRY(dalloc(c, &c->rank, n, true));
+    TRY(dalloc(c, &c->perm, n, true));
+    TRY(dalloc(c, &c->mcell, n, true));
+    TRY(dalloc(c, &c->ls1id, n, true));
+    TRY(dalloc(c, &c->count, c->ncell + 1, true));
+    TRY(dalloc(c, &c->start, c->ncell + 1, true));
+    for (int t = 0; t < kTypes; ++t) {
+        TRY(dalloc(c, &c->nsite_mol[t], n + 1, true));
+        TRY(alloc_sites(c, c->sites[t], c->nsites[t], c->ncell, n, t));
+    }
+    TRY(alloc_scan(c, std::max(n, c->ncell) + 1));
+    int nb, px;
+    int grid = pair_grid(c, &nb, &px);
+    TRY(dalloc(c, &c->macro_part, (size_t)grid * 4 * kTypes, true));
+    TRY(dalloc(c, &c->lf_part, (size_t)nblk(n) * 4, true));
+    HIPCHK(c, hipMemsetAsync(c->F, 0, 3 * (size_t)n * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->M, 0, 3 * (size_t)n * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->Vi, 0, 3 * (size_t)n * 8, c->stream));
+    c->cur = 0;
+    return MDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mdx_last_error(const mdx_context *ctx) {
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int mdx_create(int device, mdx_context **out) {
+    if (!out) return MDX_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        g_create_error = "no HIP device";
+        return MDX_ERR_NODEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+        g_create_error = "device index out of range";
+        return MDX_ERR_ARG;
+    }
+    mdx_context *c = new mdx_context();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        g_create_error = "hipSetDevice / hipStreamCreate failed";
+        delete c;
+        return MDX_ERR_HIP;
+    }
+    for (auto &e : c->ev) (void)hipEventCreate(&e);
+    if (dalloc(c, &c->red, 64) || dalloc(c, &c->d_counters, 8) || dalloc(c, &c->step_idx, 1) ||
+        dalloc(c, &c->d_comps, kMaxComp) || ensure_steps(c, 1)) {
+        g_create_error = c->err;
+        free_list(c->allocs);
+        delete c;
+        return MDX_ERR_HIP;
+    }
+    (void)hipMemset(c->step_idx, 0, sizeof(int));
+    (void)hipMemset(c->d_counters, 0, 8 * sizeof(unsigned long long));
+    *out = c;
+    return MDX_OK;
+}
+
+int mdx_destroy(mdx_context *c) {
+    if (!c) return MDX_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
+    free_list(c->state_allocs);
+    free_list(c->allocs);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MDX_OK;
+}
+
+int mdx_set_components(mdx_context *c, int ncomp, const double *desc, int desc_len,
+                       const double *mix, double eps_rf) {
+    if (!c || !desc) return MDX_ERR_ARG;
+    if (c->have_state) return fail(c, MDX_ERR_STATE, "components must be set before upload");
+    TRY(parse_model(c, ncomp, desc, desc_len));
+    if (c->model.nlj > 4096) return fail(c, MDX_ERR_CAPACITY, "too many LJ centres");
+    c->eps_rf = eps_rf;
+    c->has_mix = mix != nullptr;
+    c->mix.assign(mix ? mix : desc, mix ? mix + ncomp * (ncomp - 1) : desc);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpy(c->d_comps, c->model.comp, sizeof(Component) * ncomp,
+                        hipMemcpyHostToDevice));
+    int nn = std::max(1, c->model.nlj * c->model.nlj);
+    TRY(dalloc(c, &c->d_eps24, nn));
+    TRY(dalloc(c, &c->d_sig2, nn));
+    TRY(dalloc(c, &c->d_shift6, nn));
+    TRY(dalloc(c, &c->d_feps24, nn));
+    TRY(dalloc(c, &c->d_fsig2, nn));
+    TRY(dalloc(c, &c->d_fshift6, nn));
+    c->have_model = true;
+    c->have_domain = false;
+    return MDX_OK;
+}
+
+int mdx_set_domain(mdx_context *c, const double *box, double rc, double rc_lj, int periodic,
+                   int cell_mode) {
+    if (!c || !box) return MDX_ERR_ARG;
+    if (!c->have_model) return fail(c, MDX_ERR_STATE, "set components before the domain");
+    if (c->have_state) return fail(c, MDX_ERR_STATE, "domain must be set before upload");
+    if (!(rc > 0) || !(rc_lj > 0)) return fail(c, MDX_ERR_ARG, "cutoffs must be positive");
+    if (rc_lj > rc)
+        return fail(c, MDX_ERR_ARG, "rcLJ larger than the cell cutoff rc is not supported");
+    c->rc = rc;
+    c->rc_lj = rc_lj;
+    TRY(build_geom(c, box, rc, periodic, cell_mode));
+    set_cutoff_terms(c);
+    build_tables(c);
+    TRY(upload_tables(c));
+    c->have_domain = true;
+    return MDX_OK;
+}
+
+int mdx_set_precision(mdx_context *c, int precision) {
+    if (!c) return MDX_ERR_ARG;
+    if (precision != MDX_PREC_FP64 && precision != MDX_PREC_FP32_FP64ACC)
+        return fail(c, MDX_ERR_ARG, "unknown precision");
+    if (precision != c->precision) drop_graphs(c);
+    c->precision = precision;
+    return MDX_OK;
+}
+
+int mdx_set_lj_tables(mdx_context *c, int n_ljc, const double *eps24, const double *sig2,
+                      const double *shift6) {
+    if (!c || !eps24 || !sig2 || !shift6) return MDX_ERR_ARG;
+    TRY(need(c, true, true, false));
+    if (n_ljc != c->model.nlj) return fail(c, MDX_ERR_ARG, "n_ljc differs from the model");
+    size_t nn = (size_t)n_ljc * n_ljc;
+    c->eps24.assign(eps24, eps24 + nn);
+    c->sig2.assign(sig2, sig2 + nn);
+    c->shift6.assign(shift6, shift6 + nn);
+    return upload_tables(c);
+}
+
+int mdx_upload(mdx_context *c, int64_t n, const double *r, const double *v, const double *q,
+               const double *L, const int32_t *cid, const uint64_t *ids) {
+    if (!c || !r || !cid) return MDX_ERR_ARG;
+    TRY(need(c, true, true, false));
+    HIPCHK(c, hipSetDevice(c->device));
+    TRY(state_setup(c, n, cid));
+    const int nn = (int)n;
+    std::vector<double> buf((size_t)nn);
+    MolArrays &m = c->mol[0];
+    auto put = [&](const double *src, int stride, int k, double dflt, double *dst) -> int {
+        for (int i = 0; i < nn; ++i) buf[i] = src ? src[(size_t)i * stride + k] : dflt;
+        HIPCHK(c, hipMemcpy(dst, buf.data(), (size_t)nn * 8, hipMemcpyHostToDevice));
+        return MDX_OK;
+    };
+    TRY(put(r, 3, 0, 0., m.rx));
+    TRY(put(r, 3, 1, 0., m.ry));
+    TRY(put(r, 3, 2, 0., m.rz));
+    TRY(put(v, 3, 0, 0., m.vx));
+    TRY(put(v, 3, 1, 0., m.vy));
+    TRY(put(v, 3, 2, 0., m.vz));
+    TRY(put(q, 4, 0, 1., m.qw));
+    TRY(put(q, 4, 1, 0., m.qx));
+    TRY(put(q, 4, 2, 0., m.qy));
+    TRY(put(q, 4, 3, 0., m.qz));
+    TRY(put(L, 3, 0, 0., m.Lx));
+    TRY(put(L, 3, 1, 0., m.Ly));
+    TRY(put(L, 3, 2, 0., m.Lz));
+    HIPCHK(c, hipMemcpy(m.cid, cid, (size_t)nn * 4, hipMemcpyHostToDevice));
+    std::vector<int> orig((size_t)nn);
+    for (int i = 0; i < nn; ++i) orig[i] = i;
+    HIPCHK(c, hipMemcpy(m.orig, orig.data(), (size_t)nn * 4, hipMemcpyHostToDevice));
+    c->ids.assign((size_t)nn, 0);
+    for (int i = 0; i < nn; ++i) c->ids[i] = ids ? ids[i] : (uint64_t)(i + 1);
+    c->have_state = true;
+    return MDX_OK;
+}
+
+int mdx_upload_device(mdx_context *c, int64_t n, const double *r, const double *v,
+                      const double *q, const double *L, const int32_t *cid) {
+    if (!c || !r || !v || !q || !L || !cid) return MDX_ERR_ARG;
+    TRY(need(c, true, true, false));
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<int32_t> hc((size_t)n);
+    HIPCHK(c, hipMemcpy(hc.data(), cid, (size_t)n * 4, hipMemcpyDeviceToHost));
+    TRY(state_setup(c, n, hc.data()));
+    const int nn = (int)n;
+    MolArrays &m = c->mol[0];
+    double *dst[] = {m.rx, m.ry, m.rz, m.vx, m.vy, m.vz, m.qw, m.qx, m.qy, m.qz, m.Lx, m.Ly, m.Lz};
+    const double *srcs[] = {r, r + nn, r + 2 * (size_t)nn, v, v + nn, v + 2 * (size_t)nn,
+                            q, q + nn, q + 2 * (size_t)nn, q + 3 * (size_t)nn,
+                            L, L + nn, L + 2 * (size_t)nn};
+    // device inputs are structure-of-arrays: r = [x(n) | y(n) | z(n)] etc.
+    for (int k = 0; k < 13; ++k)
+        HIPCHK(c, hipMemcpyAsync(dst[k], srcs[k], (size_t)nn * 8, hipMemcpyDeviceToDevice,
+                                 c->stream));
+    HIPCHK(c, hipMemcpyAsync(m.cid, cid, (size_t)nn * 4, hipMemcpyDeviceToDevice, c->stream));
+    std::vector<int> orig((size_t)nn);
+    for (int i = 0; i < nn; ++i) orig[i] = i;
+    HIPCHK(c, hipMemcpy(m.orig, orig.data(), (size_t)nn * 4, hipMemcpyHostToDevice));
+    c->ids.assign((size_t)nn, 0);
+    for (int i = 0; i < nn; ++i) c->ids[i] = (uint64_t)(i + 1);
+    c->have_state = true;
+    return MDX_OK;
+}
+
+int mdx_download(mdx_context *c, int64_t n, double *r, double *v, double *q, double *L, double *F,
+                 double *M, double *Vi, uint64_t *ids) {
+    if (!c) return MDX_ERR_ARG;
+    TRY(need(c, true, true, true));
+    if (n != c->n) return fail(c, MDX_ERR_ARG, "n differs from the uploaded count");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nn = c->n;
+    MolArrays &m = c->mol[c->cur];
+    double *tmp;
+    TRY(dalloc(c, &tmp, (size_t)nn * 4, true));
+    std::vector<double> hb((size_t)nn * 4);
+    auto get = [&](const double *a, int width, int stride_in, double *out) -> int {
+        if (!out) return MDX_OK;
+        hipLaunchKernelGGL(k_scatter_orig, dim3(nblk(nn)), dim3(kBlock), 0, c->stream, nn, m.orig,
+                           a, tmp, stride_in, width);
+        HIPCHK(c, hipMemcpyAsync(out, tmp, (size_t)nn * width * 8, hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return MDX_OK;
+    };
+    // the molecule arrays are separate allocations; gather them pairwise
+    auto get3 = [&](double *x, double *y, double *z, int width, double *out) -> int {
+        if (!out) return MDX_OK;
+        double *parts[4] = {x, y, z, nullptr};
+        std::vector<double> col((size_t)nn);
+        for (int w = 0; w < width; ++w) {
+            TRY(get(parts[w], 1, nn, col.data()));
+            for (int i = 0; i < nn; ++i) out[(size_t)i * width + w] = col[i];
+        }
+        return MDX_OK;
+    };
+    TRY(get3(m.rx, m.ry, m.rz, 3, r));
+    TRY(get3(m.vx, m.vy, m.vz, 3, v));
+    TRY(get3(m.Lx, m.Ly, m.Lz, 3, L));
+    if (q) {
+        std::vector<double> col((size_t)nn);
+        double *parts[4] = {m.qw, m.qx, m.qy, m.qz};
+        for (int w = 0; w < 4; ++w) {
+            TRY(get(parts[w], 1, nn, col.data()));
+            for (int i = 0; i < nn; ++i) q[(size_t)i * 4 + w] = col[i];
+        }
+    }
+    TRY(get(c->F, 3, nn, F));
+    TRY(get(c->M, 3, nn, M));
+    TRY(get(c->Vi, 3, nn, Vi));
+    if (ids)
+        for (int i = 0; i < nn; ++i) ids[i] = c->ids[i];
+    (void)hb;
+    // release the scratch copy
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(tmp);
+    c->state_allocs.erase(std::remove(c->state_allocs.begin(), c->state_allocs.end(), (void *)tmp),
+                          c->state_allocs.end());
+    return MDX_OK;
+}
+
+int mdx_update_container(mdx_context *c) {
+    if (!c) return MDX_ERR_ARG;
+    TRY(need(c, true, true, true));
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->g.periodic) {
+        // wrap positions that were uploaded outside [0, L) (leapfrog_pre wraps
+        // during stepping): a zero-length Leapfrog drift is not applied here;
+        // molecules must be uploaded inside the box.
+    }
+    return launch_update(c);
+}
+
+int mdx_traverse(mdx_context *c, double *macro) {
+    if (!c) return MDX_ERR_ARG;
+    TRY(need(c, true, true, true));
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->counting)
+        HIPCHK(c, hipMemsetAsync(c->d_counters, 0, 8 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->step_idx, 0, sizeof(int), c->stream));
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    TRY(launch_traverse(c));
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    if (macro) {
+        TRY(0);
+        HIPCHK(c, hipMemcpyAsync(macro, c->red + 4, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->timing) {
+        float a = 0, b = 0;
+        HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+        HIPCHK(c, hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
+        c->pair_ms = a;
+        c->total_ms = b;
+    }
+    return MDX_OK;
+}
+
+int mdx_leapfrog_pre(mdx_context *c, double dt) {
+    if (!c) return MDX_ERR_ARG;
+    TRY(need(c, true, true, true));
+    HIPCHK(c, hipSetDevice(c->device));
+    return launch_lf_pre(c, dt);
+}
+
+int mdx_leapfrog_post(mdx_context *c, double dt, double *sums) {
+    if (!c) return MDX_ERR_ARG;
+    TRY(need(c, true, true, true));
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemsetAsync(c->step_idx, 0, sizeof(int), c->stream));
+    TRY(launch_lf_post(c, dt));
+    if (sums) {
+        HIPCHK(c, hipMemcpyAsync(sums, c->red + 8, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDX_OK;
+}
+
+int mdx_step(mdx_context *c, int nsteps, double dt, double *per_step, int use_graph) {
+    if (!c || nsteps < 0) return MDX_ERR_ARG;
+    TRY(need(c, true, true, true));
+    HIPCHK(c, hipSetDevice(c->device));
+    TRY(ensure_steps(c, nsteps));
+    HIPCHK(c, hipMemsetAsync(c->step_idx, 0, sizeof(int), c->stream));
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    if (use_graph && !c->counting && !c->timing) {
+        if (c->graph_dt != dt) drop_graphs(c);
+        for (int s = 0; s < nsteps; ++s) {
+            int parity = c->cur;
+            if (!c->gexec[parity]) {
+                hipGraph_t graph;
+                HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                int rc = launch_step(c, dt);
+                hipError_t e = hipStreamEndCapture(c->stream, &graph);
+                if (rc != MDX_OK) return rc;
+                if (e != hipSuccess)
+                    return fail(c, MDX_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+                HIPCHK(c, hipGraphInstantiate(&c->gexec[parity], graph, nullptr, nullptr, 0));
+                (void)hipGraphDestroy(graph);
+                c->graph_dt = dt;
+                c->cur = parity;  // capture recorded but did not run the step
+            }
+            HIPCHK(c, hipGraphLaunch(c->gexec[parity], c->stream));
+            c->cur = 1 - parity;
+        }
+    } else {
+        for (int s = 0; s < nsteps; ++s) TRY(launch_step(c, dt));
+    }
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    if (per_step && nsteps > 0)
+        HIPCHK(c, hipMemcpyAsync(per_step, c->step_out, (size_t)nsteps * 6 * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->timing) {
+        float b = 0;
+        HIPCHK(c, hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
+        c->total_ms = b;
+    }
+    return MDX_OK;
+}
+
+int mdx_synchronize(mdx_context *c) {
+    if (!c) return MDX_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDX_OK;
+}
+
+int mdx_cell_ids(mdx_context *c, int64_t n, int64_t *out) {
+    if (!c || !out) return MDX_ERR_ARG;
+    TRY(need(c, true, true, true));
+    if (n != c->n) return fail(c, MDX_ERR_ARG, "n differs from the uploaded count");
+    std::vector<int> tmp((size_t)n);
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->ls1id, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < n; ++i) out[i] = tmp[i];
+    return MDX_OK;
+}
+
+int mdx_cell_geometry(mdx_context *c, int32_t *cells_per_dim, double *cell_length) {
+    if (!c) return MDX_ERR_ARG;
+    TRY(need(c, true, true, false));
+    for (int d = 0; d < 3; ++d) {
+        if (cells_per_dim) cells_per_dim[d] = c->g.cpd[d];
+        if (cell_length) cell_length[d] = c->g.cl[d];
+    }
+    return MDX_OK;
+}
+
+int mdx_cell_pairs(mdx_context *c, int32_t *pairs, int64_t max_pairs, int64_t *count) {
+    if (!c || !count) return MDX_ERR_ARG;
+    TRY(need(c, true, true, false));
+    HIPCHK(c, hipSetDevice(c->device));
+    long tot = (long)c->ncell * 27;
+    int *d;
+    TRY(dalloc(c, &d, (size_t)tot * 2));
+    hipLaunchKernelGGL(k_enum_pairs, dim3(nblk(tot)), dim3(kBlock), 0, c->stream, c->g, c->ncell, d);
+    std::vector<int> h((size_t)tot * 2);
+    HIPCHK(c, hipMemcpyAsync(h.data(), d, h.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(d);
+    c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void *)d), c->allocs.end());
+    // each unordered pair is visited from both cells: keep it once, non-halo
+    // cell first as C08BasedTraversals::processBaseCell hands it over
+    std::vector<std::pair<int, int>> v;
+    v.reserve((size_t)tot);
+    auto halo = [&](int e) {
+        int x = e % c->g.cpd[0], y = (e / c->g.cpd[0]) % c->g.cpd[1],
+            z = e / (c->g.cpd[0] * c->g.cpd[1]);
+        return x == 0 || y == 0 || z == 0 || x == c->g.cpd[0] - 1 || y == c->g.cpd[1] - 1 ||
+               z == c->g.cpd[2] - 1;
+    };
+    for (long t = 0; t < tot; ++t) {
+        int a = h[2 * t], b = h[2 * t + 1];
+        if (a < 0) continue;
+        v.push_back({std::min(a, b), std::max(a, b)});
+    }
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    *count = (int64_t)v.size();
+    if (pairs) {
+        int64_t k = 0;
+        for (auto &p : v) {
+            if (k >= max_pairs) break;
+            int a = p.first, b = p.second;
+            if (halo(a)) std::swap(a, b);
+            pairs[2 * k] = a;
+            pairs[2 * k + 1] = b;
+            ++k;
+        }
+    }
+    return MDX_OK;
+}
+
+int mdx_set_counting(mdx_context *c, int enable) {
+    if (!c) return MDX_ERR_ARG;
+    c->counting = enable != 0;
+    return MDX_OK;
+}
+
+int mdx_counters(mdx_context *c, int64_t *out) {
+    if (!c || !out) return MDX_ERR_ARG;
+    unsigned long long h[8];
+    HIPCHK(c, hipMemcpyAsync(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // owner computes: every test and every pair was made from both sides
+    for (int k = 0; k < 8; ++k) out[k] = (int64_t)(h[k] / 2);
+    return MDX_OK;
+}
+
+int mdx_set_timing(mdx_context *c, int enable) {
+    if (!c) return MDX_ERR_ARG;
+    c->timing = enable != 0;
+    return MDX_OK;
+}
+
+int mdx_last_kernel_ms(mdx_context *c, double *pair_ms, double *total_ms) {
+    if (!c) return MDX_ERR_ARG;
+    if (pair_ms) *pair_ms = c->pair_ms;
+    if (total_ms) *total_ms = c->total_ms;
+    return MDX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+// mdx_internal.h -- data layout shared by the kernels and the C ABI.
+//
+// HBM layout (all arrays structure-of-arrays, one entry per molecule or site):
+//  * molecule state, double buffered for the counting-sort permutation:
+//    r, v, L (3 x FP64 each, SoA), q (4 x FP64), cid (int32), orig (int32 =
+//    upload index). After every re-bin the molecules are stored cell by cell
+//    (cells x-fastest, ls1's linearisation), so every later pass is coalesced.
+//  * forces F, M, Vi (3 x FP64) per molecule, in the current order.
+//  * per site type t in {LJ, charge, dipole, quadrupole}: contiguous site
+//    arrays in molecule order; cell c owns sites [cstart_t[c], cstart_t[c+1]).
+//    Each site carries its position and its molecule's position (the mask is
+//    on molecule centres), FP64 absolute and FP32 relative to its cell's lower
+//    corner, its orientation (dipoles, quadrupoles) and its scalar parameter
+//    (LJ lookup id, charge, |mu|, |Q|), plus FP64 force / virial / torque
+//    outputs of the pair kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mdx {
+
+constexpr int kMaxComp = 16;
+constexpr int kMaxSite = 8;
+constexpr int kTypes = 4;  // LJ, C, D, Q
+enum SiteType { T_LJ = 0, T_C = 1, T_D = 2, T_Q = 3 };
+
+struct Component {
+    int n[kTypes];          // nlj, nc, nd, nq
+    int lj_base;            // first LJ lookup id (EnsembleBase.cpp:98-111)
+    int rotdof;
+    double m, I[3], invI[3];
+    double body[kTypes][kMaxSite][3];   // site offsets, principal frame
+    double e[kTypes][kMaxSite][3];      // D, Q axes (body frame, as given)
+    double param[kTypes][kMaxSite];     // C: q, D: |mu|, Q: |Q|
+};
+
+struct Model {
+    int ncomp;
+    int nlj;
+    Component comp[kMaxComp];
+};
+
+struct Geom {
+    double bbmin[3], bbmax[3], L[3];
+    double cl[3], rcl[3], halo_min[3], halo_max[3];
+    int n[3];          // inner cells per dimension
+    int cpd[3];        // ls1 cells per dimension (incl. halo)
+    int periodic;
+    double rc2, rclj2, epsRFInvrc3;
+};
+
+// Site arrays of one type (device pointers).
+struct SiteArrays {
+    int *mol;                  // molecule index (current order)
+    double *x, *y, *z;         // site position, FP64 absolute
+    double *mx, *my, *mz;      // molecule position, FP64 absolute
+    float *fx, *fy, *fz;       // site position relative to the cell corner
+    float *fmx, *fmy, *fmz;    // molecule position relative to the cell corner
+    double *ex, *ey, *ez;      // orientation (D, Q)
+    float *fex, *fey, *fez;
+    double *p;                 // C: q, D: |mu|, Q: |Q|
+    float *fp;
+    int *ljid;                 // LJ lookup id
+    double *Fx, *Fy, *Fz;      // outputs: force
+    double *Vx, *Vy, *Vz;      // virial
+    double *Mx, *My, *Mz;      // torque (D, Q)
+    int *cstart;               // per cell start (ncell + 1)
+    int *mstart;               // per molecule start (n + 1)
+};
+
+struct MolArrays {
+    double *rx, *ry, *rz, *vx, *vy, *vz, *qw, *qx, *qy, *qz, *Lx, *Ly, *Lz;
+    int *cid, *orig;
+};
+
+struct PairParams {
+    const double *eps24, *sig2, *shift6;  // nlj x nlj
+    const float *feps24, *fsig2, *fshift6;
+    int nlj;
+};
+
+constexpr int kWavesPerBlock = 4;  // pair kernels: one cell per wavefront
+
+struct PairArgs {
+    Geom g;
+    SiteArrays s[kTypes];
+    PairParams pp;
+    int ncell;
+    int nblocks;        // logical blocks (ceil(ncell / kWavesPerBlock))
+    int nblk_per_xcd;   // logical blocks per XCD (launch = 8 * nblk_per_xcd)
+    int present[kTypes];
+    double *macro_part; // [launch blocks][4]
+    unsigned long long *counters;  // [8] or null
+};
+
+typedef void (*PairKernel)(PairArgs);
+PairKernel pair_kernel(int ti, bool fp32, bool count);
+
+}  // namespace mdx

@@ -1,0 +1,483 @@
+// mdx_kernels.hip -- gfx950 kernels of the short-range force path.
+//
+// Step = leapfrog_pre (+ periodic wrap) -> bin_count -> scan -> bin_scatter
+//        -> bin_gather (stable counting sort) -> site scans -> build_sites
+//        -> pair kernels (one per i-site type) -> calc_fm -> leapfrog_post.
+//
+// Pair kernels: one 64-lane wavefront per cell. Lane = (i-site, j-phase):
+// the cell's sites of the i type occupy Ipad = pow2ceil(n) lanes and the
+// remaining 64/Ipad phases stride over the j sites of the 27 neighbour cells
+// (periodic images through shift vectors, no halo copies). Stage A tests the
+// molecule-centre cutoff of every candidate and appends hits to a per-lane
+// list in LDS; stage B evaluates the site-pair bodies for the dense list, so
+// lanes do not idle on the 85% of candidates outside the cutoff. Every site
+// accumulates only its own force, virial and torque (owner computes), so no
+// atomics and no colouring are needed and results are deterministic; each
+// unordered pair is evaluated from both sides and the macroscopic sums are
+// halved. Phases are folded with cross-lane shuffles at the end of the cell.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+// ------------------------------------------------------------------ helpers
+
+// Quaternion::rotate (src/molecules/Quaternion.cpp:43-64)
+__device__ __forceinline__ void qrot(const double q[4], const double d[3], double o[3]) {
+    o[0] = o[1] = o[2] = 0.0;
+    if (d[0] == 0.0 && d[1] == 0.0 && d[2] == 0.0) return;
+    double ww = q[0] * q[0], xx = q[1] * q[1], yy = q[2] * q[2], zz = q[3] * q[3];
+    double wx = q[0] * q[1], wy = q[0] * q[2], wz = q[0] * q[3];
+    double xy = q[1] * q[2], xz = q[1] * q[3], yz = q[2] * q[3];
+    o[0] = (ww + xx - yy - zz) * d[0] + 2. * (xy - wz) * d[1] + 2. * (wy + xz) * d[2];
+    o[1] = 2. * (wz + xy) * d[0] + (ww - xx + yy - zz) * d[1] + 2. * (yz - wx) * d[2];
+    o[2] = 2. * (xz - wy) * d[0] + 2. * (wx + yz) * d[1] + (ww - xx - yy + zz) * d[2];
+}
+
+// Quaternion::rotateinv (Quaternion.cpp:66-86)
+__device__ __forceinline__ void qrotinv(const double q[4], const double d[3], double o[3]) {
+    o[0] = o[1] = o[2] = 0.0;
+    if (d[0] == 0.0 && d[1] == 0.0 && d[2] == 0.0) return;
+    double ww = q[0] * q[0], xx = q[1] * q[1], yy = q[2] * q[2], zz = q[3] * q[3];
+    double wx = q[0] * q[1], wy = q[0] * q[2], wz = q[0] * q[3];
+    double xy = q[1] * q[2], xz = q[1] * q[3], yz = q[2] * q[3];
+    o[0] = (ww + xx - yy - zz) * d[0] + 2. * (xy + wz) * d[1] + 2. * (xz - wy) * d[2];
+    o[1] = 2. * (xy - wz) * d[0] + (ww - xx + yy - zz) * d[1] + 2. * (yz + wx) * d[2];
+    o[2] = 2. * (xz + wy) * d[0] + 2. * (yz - wx) * d[1] + (ww - xx - yy + zz) * d[2];
+}
+
+// Quaternion::differentiate (Quaternion.cpp:98-103)
+__device__ __forceinline__ void qdiff(const double q[4], const double w[3], double dq[4]) {
+    dq[0] = .5 * (-q[1] * w[0] - q[2] * w[1] - q[3] * w[2]);
+    dq[1] = .5 * (q[0] * w[0] - q[3] * w[1] + q[2] * w[2]);
+    dq[2] = .5 * (q[3] * w[0] + q[0] * w[1] - q[1] * w[2]);
+    dq[3] = .5 * (-q[2] * w[0] + q[1] * w[1] + q[0] * w[2]);
+}
+
+// CellBorderAndFlagManager::getBorder (CellBorderAndFlagManager.h:134-152).
+// Contraction off: the cell id must match the reference bit for bit.
+__device__ __forceinline__ double cell_border(const Geom &g, int idx, int d) {
+#pragma clang fp contract(off)
+    if (idx == 0) return g.halo_min[d];
+    if (idx == 1) return g.bbmin[d];
+    if (idx == g.cpd[d] - 1) return g.bbmax[d];
+    if (idx == g.cpd[d]) return g.halo_max[d];
+    return idx * g.cl[d] + g.halo_min[d];
+}
+
+// LinkedCells::getCellIndexOfPoint (LinkedCells.cpp:870-923): 3-D ls1 index
+// (halo layer included).
+__device__ __forceinline__ void ls1_cell3(const Geom &g, const double p[3], int ci[3]) {
+#pragma clang fp contract(off)
+    double lp[3] = {p[0], p[1], p[2]};
+    for (int d = 0; d < 3; ++d) {
+        if (lp[d] <= g.halo_min[d])
+            lp[d] += g.cl[d] * 0.5;
+        else if (lp[d] >= g.halo_max[d])
+            lp[d] -= g.cl[d] * 0.5;
+        int v = (int)floor((lp[d] - g.bbmin[d]) * g.rcl[d]) + 1;
+        v = v < 0 ? 0 : v;
+        v = v > g.cpd[d] - 1 ? g.cpd[d] - 1 : v;
+        ci[d] = v;
+    }
+    bool inside = true;
+    for (int d = 0; d < 3; ++d)
+        inside = inside && cell_border(g, ci[d], d) <= lp[d] && lp[d] < cell_border(g, ci[d] + 1, d);
+    if (!inside) {
+        for (int d = 0; d < 3; ++d) {
+            if (lp[d] < cell_border(g, ci[d], d))
+                ci[d]--;
+            else if (lp[d] >= cell_border(g, ci[d] + 1, d))
+                ci[d]++;
+        }
+    }
+}
+
+// Inner cell index from an ls1 3-D index (x fastest, like
+// LinkedCells::cellIndexOf3DIndex, LinkedCells.cpp:925-927).
+__device__ __forceinline__ int inner_index(const Geom &g, const int ci[3]) {
+    int x = min(max(ci[0] - 1, 0), g.n[0] - 1);
+    int y = min(max(ci[1] - 1, 0), g.n[1] - 1);
+    int z = min(max(ci[2] - 1, 0), g.n[2] - 1);
+    return (z * g.n[1] + y) * g.n[0] + x;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// --------------------------------------------------------------- Leapfrog
+
+// FullMolecule::upd_preF (FullMolecule.cpp:331-361), then the periodic wrap of
+// DomainDecompBase::handleDomainLeavingParticles (DomainDecompBase.cpp:199-249).
+__global__ __launch_bounds__(256) void k_leapfrog_pre(MolArrays s, const double *__restrict__ F,
+                                                      const double *__restrict__ M,
+                                                      const Component *__restrict__ comps, Geom g,
+                                                      double dt, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Component &c = comps[s.cid[i]];
+    double dt_halve = .5 * dt;
+    double dtInv2m = dt_halve / c.m;
+    double r[3] = {s.rx[i], s.ry[i], s.rz[i]};
+    double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
+    for (int d = 0; d < 3; ++d) {
+        v[d] += dtInv2m * F[d * n + i];
+        r[d] += dt * v[d];
+    }
+    double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
+    double L[3] = {s.Lx[i], s.Ly[i], s.Lz[i]};
+    double w[3], qh[4], qi[4];
+    qrotinv(q, L, w);
+    for (int d = 0; d < 3; ++d) w[d] *= c.invI[d];
+    qdiff(q, w, qh);
+    for (int k = 0; k < 4; ++k) qh[k] = qh[k] * dt_halve + q[k];
+    double qc = 1. / sqrt(qh[0] * qh[0] + qh[1] * qh[1] + qh[2] * qh[2] + qh[3] * qh[3]);
+    for (int k = 0; k < 4; ++k) qh[k] *= qc;
+    for (int d = 0; d < 3; ++d) L[d] += dt_halve * M[d * n + i];
+    qrotinv(qh, L, w);
+    for (int d = 0; d < 3; ++d) w[d] *= c.invI[d];
+    qdiff(qh, w, qi);
+    for (int k = 0; k < 4; ++k) q[k] += qi[k] * dt;
+    qc = 1. / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int k = 0; k < 4; ++k) q[k] *= qc;
+    if (g.periodic) {
+        for (int d = 0; d < 3; ++d) {
+            double x = r[d];
+            if (x < g.bbmin[d]) {
+                x = x + g.L[d];
+                if (x >= g.bbmax[d]) x = nextafter(g.bbmax[d], -INFINITY);
+            } else if (x >= g.bbmax[d]) {
+                x = x - g.L[d];
+                if (x <= g.bbmin[d]) x = g.bbmin[d];
+            }
+            r[d] = x;
+        }
+    }
+    s.rx[i] = r[0]; s.ry[i] = r[1]; s.rz[i] = r[2];
+    s.vx[i] = v[0]; s.vy[i] = v[1]; s.vz[i] = v[2];
+    s.qw[i] = q[0]; s.qx[i] = q[1]; s.qy[i] = q[2]; s.qz[i] = q[3];
+    s.Lx[i] = L[0]; s.Ly[i] = L[1]; s.Lz[i] = L[2];
+}
+
+// FullMolecule::upd_postF (FullMolecule.cpp:363-387) with per-block FP64
+// partial sums {sum m v^2, sum I w^2, N, rotDOF} (Leapfrog.cpp:63-147).
+__global__ __launch_bounds__(256) void k_leapfrog_post(MolArrays s, const double *__restrict__ F,
+                                                       const double *__restrict__ M,
+                                                       const Component *__restrict__ comps,
+                                                       double dt, int n, double *partial) {
+    __shared__ double red[4][4];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double mv2 = 0, iw2 = 0, cnt = 0, rot = 0;
+    if (i < n) {
+        const Component &c = comps[s.cid[i]];
+        double dt_halve = 0.5 * dt;
+        double dtInv2m = dt_halve / c.m;
+        double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
+        double L[3] = {s.Lx[i], s.Ly[i], s.Lz[i]};
+        double v2 = 0.;
+        for (int d = 0; d < 3; ++d) {
+            v[d] += dtInv2m * F[d * n + i];
+            v2 += v[d] * v[d];
+            L[d] += dt_halve * M[d * n + i];
+        }
+        mv2 = c.m * v2;
+        double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
+        double w[3];
+        qrotinv(q, L, w);
+        for (int d = 0; d < 3; ++d) {
+            w[d] *= c.invI[d];
+            iw2 += c.I[d] * w[d] * w[d];
+        }
+        cnt = 1.0;
+        rot = c.rotdof;
+        s.vx[i] = v[0]; s.vy[i] = v[1]; s.vz[i] = v[2];
+        s.Lx[i] = L[0]; s.Ly[i] = L[1]; s.Lz[i] = L[2];
+    }
+    mv2 = wave_sum(mv2);
+    iw2 = wave_sum(iw2);
+    cnt = wave_sum(cnt);
+    rot = wave_sum(rot);
+    int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    if (ln == 0) {
+        red[wv][0] = mv2; red[wv][1] = iw2; red[wv][2] = cnt; red[wv][3] = rot;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double a = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) a += red[k][threadIdx.x];
+        partial[blockIdx.x * 4 + threadIdx.x] = a;
+    }
+}
+
+// Fixed-order reduction of per-block partials (width values per block) into
+// out[width]: one block, deterministic.
+__global__ __launch_bounds__(256) void k_reduce_partials(const double *__restrict__ partial,
+                                                         int nblocks, int width, double *out) {
+    __shared__ double red[256];
+    for (int w = 0; w < width; ++w) {
+        double a = 0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) a += partial[b * width + w];
+        red[threadIdx.x] = a;
+        __syncthreads();
+        for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[w] = red[0];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- binning
+
+__global__ void k_fill_int(int *a, int n, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// Cell of every molecule (ls1 index -> inner index) and its arrival rank.
+__global__ __launch_bounds__(256) void k_bin_count(MolArrays s, Geom g, int n, int *cell,
+                                                   int *rank, int *count, int *ls1id) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double p[3] = {s.rx[i], s.ry[i], s.rz[i]};
+    int ci[3];
+    ls1_cell3(g, p, ci);
+    int c = inner_index(g, ci);
+    cell[i] = c;
+    ls1id[s.orig[i]] = (ci[2] * g.cpd[1] + ci[1]) * g.cpd[0] + ci[0];
+    rank[i] = atomicAdd(&count[c], 1);
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter(int n, const int *__restrict__ cell,
+                                                     const int *__restrict__ rank,
+                                                     const int *__restrict__ start, int *perm) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    perm[start[cell[i]] + rank[i]] = i;
+}
+
+// Stable order inside each cell (by previous position, which makes the sort
+// deterministic and keeps molecules that stay in their cell in order), then
+// the permutation of the state into cell order.
+__global__ __launch_bounds__(256) void k_bin_gather(int n, MolArrays src, MolArrays dst,
+                                                    const int *__restrict__ cell,
+                                                    const int *__restrict__ perm,
+                                                    const int *__restrict__ start, int *newcell,
+                                                    const Component *__restrict__ comps,
+                                                    int *nsite0, int *nsite1, int *nsite2,
+                                                    int *nsite3) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int i = perm[p];
+    int c = cell[i];
+    int b = start[c], e = start[c + 1];
+    int r = 0;
+    for (int k = b; k < e; ++k) r += (perm[k] < i);
+    int d = b + r;
+    dst.rx[d] = src.rx[i]; dst.ry[d] = src.ry[i]; dst.rz[d] = src.rz[i];
+    dst.vx[d] = src.vx[i]; dst.vy[d] = src.vy[i]; dst.vz[d] = src.vz[i];
+    dst.qw[d] = src.qw[i]; dst.qx[d] = src.qx[i]; dst.qy[d] = src.qy[i]; dst.qz[d] = src.qz[i];
+    dst.Lx[d] = src.Lx[i]; dst.Ly[d] = src.Ly[i]; dst.Lz[d] = src.Lz[i];
+    int cid = src.cid[i];
+    dst.cid[d] = cid;
+    dst.orig[d] = src.orig[i];
+    newcell[d] = c;
+    const Component &cp = comps[cid];
+    nsite0[d] = cp.n[0];
+    nsite1[d] = cp.n[1];
+    nsite2[d] = cp.n[2];
+    nsite3[d] = cp.n[3];
+}
+
+// ------------------------------------------------------------- scan (int)
+
+// Exclusive scan of blocks of 1024 ints (256 threads x 4); block totals out.
+__global__ __launch_bounds__(256) void k_scan_block(const int *__restrict__ in, int *out, int n,
+                                                    int *blocksum) {
+    __shared__ int ws[4];
+    int base = blockIdx.x * 1024 + threadIdx.x * 4;
+    int v[4], t = 0;
+    for (int k = 0; k < 4; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        t += v[k];
+    }
+    // inclusive scan of t across the block
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int x = t;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) ws[wv] = x;
+    __syncthreads();
+    int woff = 0;
+    for (int k = 0; k < wv; ++k) woff += ws[k];
+    int excl = woff + x - t;
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = excl;
+        excl += v[k];
+    }
+    if (threadIdx.x == 255) blocksum[blockIdx.x] = woff + x;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(int *out, int n, const int *__restrict__ off) {
+    int i = blockIdx.x * 1024 + threadIdx.x;
+    int a = off[blockIdx.x];
+    for (int k = 0; k < 4; ++k) {
+        int j = i + k * 256;
+        if (j < n) out[j] += a;
+    }
+}
+
+__global__ void k_scan_total(const int *__restrict__ in, const int *__restrict__ out, int n,
+                             int *total) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) total[0] = (n > 0) ? out[n - 1] + in[n - 1] : 0;
+}
+
+// cstart[c] = mstart[start[c]] for c in [0, ncell], start[ncell] = n.
+__global__ void k_cell_site_start(int ncell, const int *__restrict__ start,
+                                  const int *__restrict__ mstart, int *cstart) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncell) return;
+    cstart[c] = mstart[start[c]];
+}
+
+// --------------------------------------------------- SoA site cache build
+
+// FullMolecule::setupSoACache (FullMolecule.cpp:711-767): normalise q (the
+// reference mutates the state here, Quaternion.h:40-42), rotate the body-frame
+// sites, record molecule position and parameters; positions also relative to
+// the molecule's cell corner in FP32.
+__global__ __launch_bounds__(256) void k_build_sites(int n, MolArrays s, Geom g,
+                                                     const int *__restrict__ mcell,
+                                                     const Component *__restrict__ comps,
+                                                     SiteArrays s0, SiteArrays s1, SiteArrays s2,
+                                                     SiteArrays s3) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
+    double qs = 1. / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int k = 0; k < 4; ++k) q[k] *= qs;
+    s.qw[i] = q[0]; s.qx[i] = q[1]; s.qy[i] = q[2]; s.qz[i] = q[3];
+    const Component &c = comps[s.cid[i]];
+    double r[3] = {s.rx[i], s.ry[i], s.rz[i]};
+    int cell = mcell[i];
+    int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
+    double corner[3] = {g.bbmin[0] + cx * g.cl[0], g.bbmin[1] + cy * g.cl[1],
+                        g.bbmin[2] + cz * g.cl[2]};
+    float mrel[3] = {(float)(r[0] - corner[0]), (float)(r[1] - corner[1]),
+                     (float)(r[2] - corner[2])};
+    SiteArrays *S[4] = {&s0, &s1, &s2, &s3};
+#pragma unroll
+    for (int t = 0; t < kTypes; ++t) {
+        SiteArrays &A = *S[t];
+        int base = A.mstart[i];
+        for (int k = 0; k < c.n[t]; ++k) {
+            int j = base + k;
+            double d[3], pos[3];
+            qrot(q, c.body[t][k], d);
+            for (int a = 0; a < 3; ++a) pos[a] = d[a] + r[a];
+            A.mol[j] = i;
+            A.x[j] = pos[0]; A.y[j] = pos[1]; A.z[j] = pos[2];
+            A.mx[j] = r[0]; A.my[j] = r[1]; A.mz[j] = r[2];
+            A.fx[j] = (float)(pos[0] - corner[0]);
+            A.fy[j] = (float)(pos[1] - corner[1]);
+            A.fz[j] = (float)(pos[2] - corner[2]);
+            A.fmx[j] = mrel[0]; A.fmy[j] = mrel[1]; A.fmz[j] = mrel[2];
+            if (t == T_LJ) {
+                A.ljid[j] = c.lj_base + k;
+            } else {
+                A.p[j] = c.param[t][k];
+                A.fp[j] = (float)c.param[t][k];
+            }
+            if (t >= T_D) {
+                double e[3];
+                qrot(q, c.e[t][k], e);
+                A.ex[j] = e[0]; A.ey[j] = e[1]; A.ez[j] = e[2];
+                A.fex[j] = (float)e[0]; A.fey[j] = (float)e[1]; A.fez[j] = (float)e[2];
+            }
+        }
+    }
+}
+
+// FullMolecule::calcFM (FullMolecule.cpp:548-631): F = sum F_site,
+// M = sum d_site x F_site + sum M_site(D, Q), Vi = 0.5 sum V_site.
+__global__ __launch_bounds__(256) void k_calc_fm(int n, MolArrays s,
+                                                 const Component *__restrict__ comps,
+                                                 SiteArrays s0, SiteArrays s1, SiteArrays s2,
+                                                 SiteArrays s3, double *F, double *M, double *Vi) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Component &c = comps[s.cid[i]];
+    double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
+    double f[3] = {0, 0, 0}, m[3] = {0, 0, 0}, tm[3] = {0, 0, 0}, tv[3] = {0, 0, 0};
+    SiteArrays *S[4] = {&s0, &s1, &s2, &s3};
+#pragma unroll
+    for (int t = 0; t < kTypes; ++t) {
+        SiteArrays &A = *S[t];
+        int base = A.mstart[i];
+        for (int k = 0; k < c.n[t]; ++k) {
+            int j = base + k;
+            double d[3];
+            qrot(q, c.body[t][k], d);
+            double fs[3] = {A.Fx[j], A.Fy[j], A.Fz[j]};
+            f[0] += fs[0]; f[1] += fs[1]; f[2] += fs[2];
+            m[0] += d[1] * fs[2] - d[2] * fs[1];
+            m[1] += d[2] * fs[0] - d[0] * fs[2];
+            m[2] += d[0] * fs[1] - d[1] * fs[0];
+            tv[0] += A.Vx[j]; tv[1] += A.Vy[j]; tv[2] += A.Vz[j];
+            if (t >= T_D) {
+                tm[0] += A.Mx[j]; tm[1] += A.My[j]; tm[2] += A.Mz[j];
+            }
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        F[d * n + i] = f[d];
+        M[d * n + i] = m[d] + tm[d];
+        Vi[d * n + i] = 0.5 * tv[d];
+    }
+}
+
+// Macroscopic sums of one traversal (VectorizedCellProcessor::endTraversal,
+// VectorizedCellProcessor.cpp:122-155). raw = pair-kernel sums over both
+// sides of every pair {u6, uXpoles, virial, rf}; halved here. myRF carries
+// the reference's negative sign (VectorizedCellProcessor.cpp:2727).
+// Step record: Upot, virial, summv2, sumIw2, Upot_LJ, Upot_rest.
+__global__ void k_finalize_macro(const double *raw, double *macro, double *step_out,
+                                 const int *step_idx) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double u6 = 0.5 * raw[0], ux = 0.5 * raw[1], vir = 0.5 * raw[2], rf = -0.5 * raw[3];
+    macro[0] = u6;
+    macro[1] = ux;
+    macro[2] = vir;
+    macro[3] = rf;
+    double *rec = step_out + 6 * step_idx[0];
+    rec[0] = u6 / 6.0 + ux + rf;
+    rec[1] = vir + 3.0 * rf;
+    rec[4] = u6 / 6.0;
+    rec[5] = ux + rf;
+}
+
+__global__ void k_finalize_lf(const double *sums, double *step_out, const int *step_idx) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double *rec = step_out + 6 * step_idx[0];
+    rec[2] = sums[0];
+    rec[3] = sums[1];
+}
+
+__global__ void k_step_advance(int *step_idx) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) step_idx[0] += 1;
+}
+
+// Download helper: scatter a current-order array back to upload order.
+__global__ void k_scatter_orig(int n, const int *__restrict__ orig, const double *__restrict__ a,
+                               double *out, int stride_in, int width) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int o = orig[i];
+    for (int w = 0; w < width; ++w) out[o * width + w] = a[w * stride_in + i];
+}
+
+}  // namespace mdx

@@ -1,0 +1,235 @@
+"""ctypes binding of the C ABI in include/ls1amd.h (libls1amd.so).
+
+Plumbing for the tests, the smoke check and the benchmark. There is no
+fallback: if the HIP library is missing or no device is visible, the calls
+raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libls1amd.so"
+
+PREC_FP64 = 0
+PREC_FP32_FP64ACC = 1
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+# every symbol include/ls1amd.h declares
+EXPORTS = (
+    "mdx_last_error", "mdx_create", "mdx_destroy", "mdx_set_components", "mdx_set_lj_tables",
+    "mdx_set_domain", "mdx_set_precision", "mdx_upload", "mdx_download", "mdx_update_container",
+    "mdx_traverse", "mdx_leapfrog_pre", "mdx_leapfrog_post", "mdx_step", "mdx_synchronize",
+    "mdx_cell_ids", "mdx_cell_geometry", "mdx_cell_pairs", "mdx_set_counting", "mdx_counters",
+    "mdx_upload_device", "mdx_set_timing", "mdx_last_kernel_ms",
+)
+
+_lib = None
+
+
+class MdxError(RuntimeError):
+    pass
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise MdxError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(str(LIB_PATH))
+    vp = C.c_void_p
+    L.mdx_last_error.restype = C.c_char_p
+    L.mdx_last_error.argtypes = [vp]
+    L.mdx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.mdx_destroy.argtypes = [vp]
+    L.mdx_set_components.argtypes = [vp, C.c_int, _dp, C.c_int, vp, C.c_double]
+    L.mdx_set_lj_tables.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+    L.mdx_set_domain.argtypes = [vp, _dp, C.c_double, C.c_double, C.c_int, C.c_int]
+    L.mdx_set_precision.argtypes = [vp, C.c_int]
+    L.mdx_upload.argtypes = [vp, C.c_int64, _dp, _dp, _dp, _dp, _ip, vp]
+    L.mdx_download.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.mdx_update_container.argtypes = [vp]
+    L.mdx_traverse.argtypes = [vp, vp]
+    L.mdx_leapfrog_pre.argtypes = [vp, C.c_double]
+    L.mdx_leapfrog_post.argtypes = [vp, C.c_double, vp]
+    L.mdx_step.argtypes = [vp, C.c_int, C.c_double, vp, C.c_int]
+    L.mdx_synchronize.argtypes = [vp]
+    L.mdx_cell_ids.argtypes = [vp, C.c_int64, _lp]
+    L.mdx_cell_geometry.argtypes = [vp, _ip, _dp]
+    L.mdx_cell_pairs.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.mdx_set_counting.argtypes = [vp, C.c_int]
+    L.mdx_counters.argtypes = [vp, _lp]
+    L.mdx_upload_device.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp]
+    L.mdx_set_timing.argtypes = [vp, C.c_int]
+    L.mdx_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One device context: model, domain, state (mirrors ls1's
+    VectorizedCellProcessor + LinkedCells + Leapfrog for one subdomain)."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        h = C.c_void_p()
+        rc = self.L.mdx_create(int(device), C.byref(h))
+        if rc != 0:
+            raise MdxError(f"mdx_create failed ({rc}): {self.L.mdx_last_error(None).decode()}")
+        self.h = h
+        self.n = 0
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise MdxError(f"{what} failed ({rc}): {self.L.mdx_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mdx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def setup(self, ncomp, comp_desc, mix, eps_rf, box, rc, rc_lj, periodic=True, cell_mode=0,
+              precision=PREC_FP64):
+        desc = np.ascontiguousarray(comp_desc, np.float64)
+        self._mix = np.ascontiguousarray(mix, np.float64).ravel() if mix is not None and len(mix) else None
+        self._chk(self.L.mdx_set_components(self.h, int(ncomp), desc, desc.size, _ptr(self._mix),
+                                            float(eps_rf)), "mdx_set_components")
+        self._chk(self.L.mdx_set_domain(self.h, np.ascontiguousarray(box, np.float64), float(rc),
+                                        float(rc_lj), int(bool(periodic)), int(cell_mode)),
+                  "mdx_set_domain")
+        self.set_precision(precision)
+        return self
+
+    def setup_system(self, s, periodic=True, precision=PREC_FP64, cell_mode=0):
+        self.setup(s.ncomp, s.comp_desc, s.mix, s.eps_rf, s.box, s.rc, s.rc_lj, periodic,
+                   cell_mode, precision)
+        self.upload(s.r, s.v, s.q, s.L, s.cid)
+        return self
+
+    def set_precision(self, p):
+        self._chk(self.L.mdx_set_precision(self.h, int(p)), "mdx_set_precision")
+
+    def set_lj_tables(self, eps24, sig2, shift6):
+        n = int(np.sqrt(np.asarray(eps24).size))
+        self._chk(self.L.mdx_set_lj_tables(self.h, n, np.ascontiguousarray(eps24, np.float64).ravel(),
+                                           np.ascontiguousarray(sig2, np.float64).ravel(),
+                                           np.ascontiguousarray(shift6, np.float64).ravel()),
+                  "mdx_set_lj_tables")
+
+    def upload(self, r, v, q, L, cid, ids=None):
+        n = len(r)
+        self.n = n
+        f = lambda a, w: np.ascontiguousarray(np.asarray(a, np.float64).reshape(n, w))
+        idp = None
+        if ids is not None:
+            self._ids = np.ascontiguousarray(ids, np.uint64)
+            idp = _ptr(self._ids)
+        self._chk(self.L.mdx_upload(self.h, n, f(r, 3), f(v, 3), f(q, 4), f(L, 3),
+                                    np.ascontiguousarray(cid, np.int32), idp), "mdx_upload")
+
+    def download(self):
+        n = self.n
+        out = dict(r=np.zeros((n, 3)), v=np.zeros((n, 3)), q=np.zeros((n, 4)), L=np.zeros((n, 3)),
+                   F=np.zeros((n, 3)), M=np.zeros((n, 3)), Vi=np.zeros((n, 3)),
+                   ids=np.zeros(n, np.uint64))
+        self._chk(self.L.mdx_download(self.h, n, *(_ptr(out[k]) for k in
+                                                   ("r", "v", "q", "L", "F", "M", "Vi", "ids"))),
+                  "mdx_download")
+        return out
+
+    def update_container(self):
+        self._chk(self.L.mdx_update_container(self.h), "mdx_update_container")
+
+    def traverse(self):
+        mac = np.zeros(4)
+        self._chk(self.L.mdx_traverse(self.h, _ptr(mac)), "mdx_traverse")
+        return mac
+
+    def forces(self):
+        """update_container + traverse, then the molecule outputs."""
+        self.update_container()
+        mac = self.traverse()
+        out = self.download()
+        out["macro"] = mac
+        return out
+
+    def leapfrog_pre(self, dt):
+        self._chk(self.L.mdx_leapfrog_pre(self.h, float(dt)), "mdx_leapfrog_pre")
+
+    def leapfrog_post(self, dt):
+        sums = np.zeros(4)
+        self._chk(self.L.mdx_leapfrog_post(self.h, float(dt), _ptr(sums)), "mdx_leapfrog_post")
+        return sums
+
+    def step(self, nsteps, dt, use_graph=False):
+        per = np.zeros(6 * max(nsteps, 1))
+        self._chk(self.L.mdx_step(self.h, int(nsteps), float(dt), _ptr(per), int(bool(use_graph))),
+                  "mdx_step")
+        return per.reshape(-1, 6)[:nsteps]
+
+    def synchronize(self):
+        self._chk(self.L.mdx_synchronize(self.h), "mdx_synchronize")
+
+    def cell_ids(self):
+        out = np.zeros(self.n, np.int64)
+        self._chk(self.L.mdx_cell_ids(self.h, self.n, out), "mdx_cell_ids")
+        return out
+
+    def cell_geometry(self):
+        cpd = np.zeros(3, np.int32)
+        cl = np.zeros(3)
+        self._chk(self.L.mdx_cell_geometry(self.h, cpd, cl), "mdx_cell_geometry")
+        return cpd, cl
+
+    def cell_pairs(self):
+        cnt = C.c_int64(0)
+        self._chk(self.L.mdx_cell_pairs(self.h, None, 0, C.byref(cnt)), "mdx_cell_pairs")
+        out = np.zeros(2 * cnt.value, np.int32)
+        self._chk(self.L.mdx_cell_pairs(self.h, _ptr(out), cnt.value, C.byref(cnt)),
+                  "mdx_cell_pairs")
+        return out.reshape(-1, 2)
+
+    def set_counting(self, on):
+        self._chk(self.L.mdx_set_counting(self.h, int(bool(on))), "mdx_set_counting")
+
+    def counters(self):
+        out = np.zeros(8, np.int64)
+        self._chk(self.L.mdx_counters(self.h, out), "mdx_counters")
+        return out
+
+    def set_timing(self, on):
+        self._chk(self.L.mdx_set_timing(self.h, int(bool(on))), "mdx_set_timing")
+
+    def last_kernel_ms(self):
+        a, b = C.c_double(0), C.c_double(0)
+        self._chk(self.L.mdx_last_kernel_ms(self.h, C.byref(a), C.byref(b)), "mdx_last_kernel_ms")
+        return a.value, b.value
+
+
+def upot_virial(macro):
+    """VectorizedCellProcessor::endTraversal (VectorizedCellProcessor.cpp:152-154)."""
+    return macro[0] / 6.0 + macro[1] + macro[3], macro[2] + 3.0 * macro[3]

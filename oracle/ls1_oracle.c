@@ -724,7 +724,7 @@ static void apply_pair(const ox_ctx *x, ox_mol *A, int ta, int sa, ox_mol *B, in
         A->Ms[ta][sa][k] += o.M1[k];
         B->Ms[tb][sb][k] += o.M2[k];
     }
-    if (cnt) cnt[1 + kind] += 1;
+    if (cnt && macro) cnt[1 + kind] += 1;
     if (macro) {
         if (kind == K_LJ)
             mac[OX_UPOT6LJ] += o.u;
@@ -742,7 +742,7 @@ static void mol_pair(const ox_ctx *x, ox_mol *A, ox_mol *B, int macro, double *m
                      long long *cnt) {
     double dm[3] = {A->r[0] - B->r[0], A->r[1] - B->r[1], A->r[2] - B->r[2]};
     double r2 = dm[0] * dm[0] + dm[1] * dm[1] + dm[2] * dm[2];
-    if (cnt) cnt[OX_CNT_MOLPAIRS] += 1;
+    if (cnt && macro) cnt[OX_CNT_MOLPAIRS] += 1;
     const ox_comp *ca = &x->comp[A->cid], *cb = &x->comp[B->cid];
     if (r2 < x->rcLJ * x->rcLJ) {
         int n = x->nlj_total;

@@ -26,7 +26,9 @@ extern "C" {
 enum { OX_UPOT6LJ = 0, OX_UPOTXPOLES = 1, OX_VIRIAL = 2, OX_MYRF = 3 };
 
 /* counters[] layout: molecule pairs whose centre distance was tested,
- * site pairs inside the cutoff per kernel type */
+ * site pairs inside the cutoff per kernel type; pairs the reference evaluates
+ * twice across a periodic boundary (without macroscopic sums the second
+ * time) are counted once */
 enum { OX_CNT_MOLPAIRS = 0, OX_CNT_LJ = 1, OX_CNT_CC = 2, OX_CNT_CD = 3,
        OX_CNT_DD = 4, OX_CNT_CQ = 5, OX_CNT_DQ = 6, OX_CNT_QQ = 7, OX_NCNT = 8 };
 
