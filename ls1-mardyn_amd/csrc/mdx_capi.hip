@@ -208,7 +208,9 @@ int parse_model(mdx_context *c, int ncomp, const double *desc, int len) {
         cp.rotdof = rotdof;
         cp.lj_base = base;
         base += cp.n[0];
-        if (!(cp.m > 0.)) return fail(c, MDX_ERR_ARG, "component has no mass");
+        // massless components (only dipoles / quadrupoles) are valid for the
+        // force traversal; FullMolecule::upd_preF asserts m > 0 only when
+        // such a molecule is integrated (FullMolecule.cpp:332)
     }
     m.nlj = base;
     return MDX_OK;

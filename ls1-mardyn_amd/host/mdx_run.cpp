@@ -1,0 +1,107 @@
+// mdx_run -- command-line driver of the device path, in the role of
+// ls1-MarDyn's MarDyn executable for the force path: it reads an ls1 ASCII
+// phase-space file, builds HIPLinkedCells + HIPCellProcessor + HIPLeapfrog
+// (what <datastructure type="HIPLinkedCells"> selects), runs
+// Simulation::prepare_start's initial force evaluation and then the step
+// order of Simulation::simulateOneTimestep (Simulation.cpp:1076-1307) without
+// thermostat, long-range correction or plugins, and prints the reference's
+// per-step line (Simulation.cpp:1383-1386).
+//
+//   mdx_run <file.inp> --cutoff RC [--lj-cutoff RCLJ] [--dt DT] [--steps N]
+//           [--precision fp64|fp32] [--open-boundary] [--checkpoint OUT.inp]
+//           [--read-write-only OUT.inp]
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <string>
+
+#include "HIPLinkedCells.h"
+#include "PhaseSpaceIO.h"
+
+using namespace ls1amd;
+
+static void usage() {
+    std::fprintf(stderr,
+                 "usage: mdx_run <file.inp> --cutoff RC [--lj-cutoff RCLJ] [--dt DT] "
+                 "[--steps N] [--precision fp64|fp32] [--open-boundary] "
+                 "[--checkpoint OUT.inp] [--read-write-only OUT.inp]\n");
+}
+
+int main(int argc, char **argv) {
+    if (argc < 2 || !std::strcmp(argv[1], "--help") || !std::strcmp(argv[1], "-h")) {
+        usage();
+        return argc < 2 ? 1 : 0;
+    }
+    std::string inp = argv[1], checkpoint, rwonly;
+    double rc = -1, rclj = -1, dt = 0.002;
+    long steps = 0;
+    int precision = MDX_PREC_FP64;
+    bool periodic = true;
+    for (int i = 2; i < argc; ++i) {
+        std::string a = argv[i];
+        auto val = [&]() -> std::string {
+            if (i + 1 >= argc) {
+                usage();
+                std::exit(1);
+            }
+            return argv[++i];
+        };
+        if (a == "--cutoff") rc = std::stod(val());
+        else if (a == "--lj-cutoff") rclj = std::stod(val());
+        else if (a == "--dt") dt = std::stod(val());
+        else if (a == "--steps") steps = std::stol(val());
+        else if (a == "--precision") precision = val() == "fp32" ? MDX_PREC_FP32_FP64ACC : MDX_PREC_FP64;
+        else if (a == "--open-boundary") periodic = false;
+        else if (a == "--checkpoint") checkpoint = val();
+        else if (a == "--read-write-only") rwonly = val();
+        else {
+            usage();
+            return 1;
+        }
+    }
+    try {
+        PhaseSpace ps = readPhaseSpace(inp);
+        if (!rwonly.empty()) {
+            writePhaseSpace(rwonly, ps);
+            return 0;
+        }
+        if (rc <= 0) {
+            usage();
+            return 1;
+        }
+        if (rclj <= 0) rclj = rc;
+        double bmin[3] = {0, 0, 0}, bmax[3] = {ps.box[0], ps.box[1], ps.box[2]};
+        HIPLinkedCells cells(bmin, bmax, rc, rclj, ps.model, periodic, 0, precision);
+        Domain domain(ps.box);
+        HIPCellProcessor proc(domain, rc, rclj);
+        HIPLeapfrog integrator(dt);
+        cells.addParticles(ps.molecules);
+        domain.setglobalNumMolecules(ps.molecules.size());
+        // Simulation::prepare_start (Simulation.cpp:847-999)
+        cells.update();
+        cells.traverseCells(proc);
+        std::cout.precision(12);
+        std::cout << "Simstep = 0\tU_pot = " << domain.getAverageGlobalUpot()
+                  << "\tvirial = " << domain.getLocalVirial() << std::endl;
+        for (long s = 1; s <= steps; ++s) {
+            integrator.eventNewTimestep(&cells, &domain);
+            cells.update();
+            cells.traverseCells(proc);
+            integrator.eventForcesCalculated(&cells, &domain);
+            std::cout << "Simstep = " << s << "\tT = " << domain.getGlobalTemperature()
+                      << "\tU_pot = " << domain.getAverageGlobalUpot()
+                      << "\tp = " << domain.getGlobalPressure() << std::endl;
+        }
+        if (!checkpoint.empty()) {
+            PhaseSpace out = ps;
+            out.molecules = cells.getMolecules();
+            out.currentTime = ps.currentTime + steps * dt;
+            writePhaseSpace(checkpoint, out);
+        }
+    } catch (const std::exception &e) {
+        std::cerr << "mdx_run: " << e.what() << std::endl;
+        return 2;
+    }
+    return 0;
+}

@@ -118,3 +118,32 @@ def read_inp(path) -> PhaseSpace:
     return PhaseSpace(box, temperature, ncomp, np.asarray(desc, np.float64),
                       np.asarray(mix, np.float64).reshape(-1, 2), eps_rf,
                       ids, cid, r, v, q, L)
+
+
+def write_inp(path, box, ncomp, comp_desc, mix, eps_rf, ids, cid, r, v, q, L, temperature=0.0):
+    """Writes the ICRVQD format ASCIIReader reads (1-based component ids)."""
+    desc = [float(x) for x in comp_desc]
+    lines = ["mardyn trunk 20100525", "currentTime\t0",
+             " Length\t%.17g %.17g %.17g" % tuple(box), f" Temperature\t{temperature!r}",
+             f" NumberOfComponents\t{ncomp}"]
+    p = 0
+    for _ in range(ncomp):
+        n = [int(x) for x in desc[p:p + 4]]
+        lines.append("\t".join(str(k) for k in n) + "\t0")
+        p += 4
+        for width, count in zip((8, 5, 7, 7), n):
+            for _ in range(count):
+                lines.append(" ".join("%.17g" % x for x in desc[p:p + width]))
+                p += width
+        lines.append(" ".join("%.17g" % x for x in desc[p:p + 3]))
+        p += 3
+    for xi, eta in np.asarray(mix, np.float64).reshape(-1, 2):
+        lines.append("%.17g %.17g" % (xi, eta))
+    lines.append("%.17g" % eps_rf)
+    lines.append(f" NumberOfMolecules\t{len(r)}")
+    lines.append(" MoleculeFormat\tICRVQD")
+    for i in range(len(r)):
+        vals = list(r[i]) + list(v[i]) + list(q[i]) + list(L[i])
+        lines.append(f"{int(ids[i])}\t{int(cid[i]) + 1}\t" + "\t".join("%.17g" % x for x in vals))
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")

@@ -1,0 +1,56 @@
+"""The C++ host layer (HIPLinkedCells / HIPCellProcessor / HIPLeapfrog over
+the C ABI) and its driver mdx_run."""
+import pathlib
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import scenarios
+from ls1inp import read_inp, write_inp
+from oracle_lib import Oracle
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+MDX_RUN = ROOT / "ls1-mardyn_amd" / "mdx_run"
+GOLD = ROOT / "tests" / "golden"
+
+
+def test_driver_usage():
+    r = subprocess.run([str(MDX_RUN), "--help"], capture_output=True, text=True)
+    assert r.returncode == 0 and "usage" in r.stderr
+
+
+@pytest.mark.parametrize("name", ["VectorizationMultiComponentMultiPotentials_50_molecules",
+                                  "VectorizationWater", "ForceCalculationTestU0"])
+def test_phase_space_round_trip(name, tmp_path):
+    src = GOLD / "inp" / f"{name}.inp"
+    out = tmp_path / "rt.inp"
+    subprocess.run([str(MDX_RUN), str(src), "--read-write-only", str(out)], check=True)
+    a, b = read_inp(src), read_inp(out)
+    for k in ("box", "comp_desc", "mix", "ids", "cid", "r", "v", "q", "L"):
+        np.testing.assert_array_equal(getattr(a, k), getattr(b, k), err_msg=k)
+    assert a.eps_rf == b.eps_rf and a.ncomp == b.ncomp
+
+
+@pytest.mark.gpu
+def test_driver_trajectory_vs_oracle(tmp_path):
+    s = scenarios.small_mixture(seed=71)
+    f = tmp_path / "mix.inp"
+    write_inp(f, s.box, s.ncomp, s.comp_desc, s.mix, s.eps_rf, np.arange(1, s.n + 1), s.cid, s.r,
+              s.v, s.q, s.L)
+    r = subprocess.run([str(MDX_RUN), str(f), "--cutoff", str(s.rc), "--dt", "0.001", "--steps",
+                        "20", "--checkpoint", str(tmp_path / "out.inp")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    upot = [float(m) for m in re.findall(r"U_pot = ([-0-9.eE+]+)", r.stdout)]
+    assert len(upot) == 21
+    o = Oracle(s.ncomp, s.comp_desc, s.mix, s.eps_rf, s.rc, s.rc, s.box, periodic=True)
+    ref = o.simulate(20, 0.001, s.r, s.v, s.q, s.L, s.cid)
+    ref_u = np.concatenate([[sum(ref["initial"]["macro"][[0]]) / 6 + ref["initial"]["macro"][1]
+                             + ref["initial"]["macro"][3]], ref["per_step"][:, 0]]) / s.n
+    np.testing.assert_allclose(upot, ref_u, rtol=1e-9)
+    fin = read_inp(tmp_path / "out.inp")
+    d = fin.r - ref["r"]
+    d -= np.round(d / s.box) * s.box
+    assert np.abs(d).max() < 1e-9
