@@ -16,12 +16,13 @@ __global__ void k_leapfrog_pre(MolArrays, const double *, const double *, const 
                                double, int);
 __global__ void k_leapfrog_post(MolArrays, const double *, const double *, const Component *,
                                 double, int, double *);
-__global__ void k_reduce_partials(const double *, int, int, double *);
+__global__ void k_reduce_partials(const double *, int, int, int, double *);
 __global__ void k_fill_int(int *, int, int);
 __global__ void k_bin_count(MolArrays, Geom, int, int *, int *, int *, int *);
 __global__ void k_bin_scatter(int, const int *, const int *, const int *, int *);
 __global__ void k_bin_gather(int, MolArrays, MolArrays, const int *, const int *, const int *,
-                             int *, const Component *, int *, int *, int *, int *);
+                             int *, const int *, int *, const Component *, int *, int *, int *,
+                             int *);
 __global__ void k_scan_block(const int *, int *, int, int *);
 __global__ void k_scan_add(int *, int, const int *);
 __global__ void k_scan_total(const int *, const int *, int, int *);
@@ -57,6 +58,7 @@ struct mdx_context {
     Component *d_comps = nullptr;
     std::vector<double> site_eps, site_sig, site_shift6, mix;
     bool has_mix = false;
+    bool lj_at_origin = false;  // every LJ site sits at its molecule's centre
     std::vector<double> eps24, sig2, shift6;
     double *d_eps24 = nullptr, *d_sig2 = nullptr, *d_shift6 = nullptr;
     float *d_feps24 = nullptr, *d_fsig2 = nullptr, *d_fshift6 = nullptr;
@@ -72,7 +74,8 @@ struct mdx_context {
     int cur = 0;
     double *F = nullptr, *M = nullptr, *Vi = nullptr;
     int *cell = nullptr, *rank = nullptr, *count = nullptr, *start = nullptr, *perm = nullptr,
-        *mcell = nullptr, *ls1id = nullptr;
+        *mcell = nullptr, *ls1id = nullptr, *mls1id = nullptr;
+    double *red_tmp = nullptr;  // first level of the two-level reductions
     int *nsite_mol[kTypes] = {nullptr, nullptr, nullptr, nullptr};
     SiteArrays sites[kTypes]{};
     std::vector<int *> scan_tmp;
@@ -213,6 +216,11 @@ int parse_model(mdx_context *c, int ncomp, const double *desc, int len) {
         // such a molecule is integrated (FullMolecule.cpp:332)
     }
     m.nlj = base;
+    c->lj_at_origin = true;
+    for (int ci = 0; ci < ncomp; ++ci)
+        for (int s = 0; s < m.comp[ci].n[0]; ++s)
+            for (int d = 0; d < 3; ++d)
+                if (m.comp[ci].body[0][s][d] != 0.0) c->lj_at_origin = false;
     return MDX_OK;
 }
 
@@ -390,9 +398,9 @@ int scan_exclusive(mdx_context *c, const int *in, int *out, int n, int level = 0
     return MDX_OK;
 }
 
-int pair_grid(const mdx_context *c, int *nblocks, int *per_xcd) {
-    *nblocks = (c->ncell + kWavesPerBlock - 1) / kWavesPerBlock;
-    *per_xcd = (*nblocks + 7) / 8;
+// one 64-lane block per cell, launched as 8 contiguous ranges (one per XCD)
+int pair_grid(const mdx_context *c, int *per_xcd) {
+    *per_xcd = (c->ncell + 7) / 8;
     return 8 * *per_xcd;
 }
 
@@ -400,6 +408,21 @@ int n_present(const mdx_context *c) {
     int k = 0;
     for (int t = 0; t < kTypes; ++t) k += c->nsites[t] > 0;
     return k;
+}
+
+// Sum of nb blocks x width partials into out[width], fixed order.
+void launch_reduce(mdx_context *c, const double *in, int nb, int width, double *out) {
+    const int g1 = 256;
+    if (nb <= 4 * g1) {
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->stream, in, nb, width, nb,
+                           out);
+        return;
+    }
+    const int per = (nb + g1 - 1) / g1;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(g1), dim3(256), 0, c->stream, in, nb, width, per,
+                       c->red_tmp);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->stream, c->red_tmp, g1, width,
+                       g1, out);
 }
 
 // Re-bin (LinkedCells::update + exchangeMolecules' periodic wrap, which
@@ -415,7 +438,8 @@ int launch_update(mdx_context *c) {
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, c->cell,
                        c->rank, c->start, c->perm);
     hipLaunchKernelGGL(k_bin_gather, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, src, dst,
-                       c->cell, c->perm, c->start, c->mcell, c->d_comps, c->nsite_mol[0],
+                       c->cell, c->perm, c->start, c->mcell, c->ls1id, c->mls1id, c->d_comps,
+                       c->nsite_mol[0],
                        c->nsite_mol[1], c->nsite_mol[2], c->nsite_mol[3]);
     c->cur = 1 - c->cur;
     for (int t = 0; t < kTypes; ++t) {
@@ -447,20 +471,20 @@ int launch_traverse(mdx_context *c) {
     a.pp.fshift6 = c->d_fshift6;
     a.pp.nlj = c->model.nlj;
     a.ncell = c->ncell;
-    int grid = pair_grid(c, &a.nblocks, &a.nblk_per_xcd);
+    int grid = pair_grid(c, &a.nblk_per_xcd);
     a.counters = c->d_counters;
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     int slot = 0;
     for (int t = 0; t < kTypes; ++t) {
         if (c->nsites[t] == 0) continue;
         a.macro_part = c->macro_part + (size_t)slot * grid * 4;
-        PairKernel k = pair_kernel(t, c->precision == MDX_PREC_FP32_FP64ACC, c->counting);
-        hipLaunchKernelGGL(k, dim3(grid), dim3(64 * kWavesPerBlock), 0, c->stream, a);
+        PairKernel k = pair_kernel(t, c->precision == MDX_PREC_FP32_FP64ACC, c->counting,
+                                   t == T_LJ && c->lj_at_origin);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, c->stream, a);
         ++slot;
     }
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->stream, c->macro_part,
-                       slot * grid, 4, c->red);
+    launch_reduce(c, c->macro_part, slot * grid, 4, c->red);
     hipLaunchKernelGGL(k_calc_fm, dim3(nblk(c->n)), dim3(kBlock), 0, c->stream, c->n,
                        c->mol[c->cur], c->d_comps, c->sites[0], c->sites[1], c->sites[2],
                        c->sites[3], c->F, c->M, c->Vi);
@@ -481,8 +505,7 @@ int launch_lf_post(mdx_context *c, double dt) {
     int nb = nblk(c->n);
     hipLaunchKernelGGL(k_leapfrog_post, dim3(nb), dim3(kBlock), 0, c->stream, c->mol[c->cur], c->F,
                        c->M, c->d_comps, dt, c->n, c->lf_part);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->stream, c->lf_part, nb, 4,
-                       c->red + 8);
+    launch_reduce(c, c->lf_part, nb, 4, c->red + 8);
     hipLaunchKernelGGL(k_finalize_lf, dim3(1), dim3(1), 0, c->stream, c->red + 8, c->step_out,
                        c->step_idx);
     HIPCHK(c, hipGetLastError());
@@ -543,6 +566,7 @@ int state_setup(mdx_context *c, int64_t n64, const int32_t *cid_host) {
     TRY(dalloc(c, &c->perm, n, true));
     TRY(dalloc(c, &c->mcell, n, true));
     TRY(dalloc(c, &c->ls1id, n, true));
+    TRY(dalloc(c, &c->mls1id, n, true));
     TRY(dalloc(c, &c->count, c->ncell + 1, true));
     TRY(dalloc(c, &c->start, c->ncell + 1, true));
     for (int t = 0; t < kTypes; ++t) {
@@ -550,8 +574,8 @@ int state_setup(mdx_context *c, int64_t n64, const int32_t *cid_host) {
         TRY(alloc_sites(c, c->sites[t], c->nsites[t], c->ncell, n, t));
     }
     TRY(alloc_scan(c, std::max(n, c->ncell) + 1));
-    int nb, px;
-    int grid = pair_grid(c, &nb, &px);
+    int px;
+    int grid = pair_grid(c, &px);
     TRY(dalloc(c, &c->macro_part, (size_t)grid * 4 * kTypes, true));
     TRY(dalloc(c, &c->lf_part, (size_t)nblk(n) * 4, true));
     HIPCHK(c, hipMemsetAsync(c->F, 0, 3 * (size_t)n * 8, c->stream));
@@ -589,7 +613,7 @@ int mdx_create(int device, mdx_context **out) {
         return MDX_ERR_HIP;
     }
     for (auto &e : c->ev) (void)hipEventCreate(&e);
-    if (dalloc(c, &c->red, 64) || dalloc(c, &c->d_counters, 8) || dalloc(c, &c->step_idx, 1) ||
+    if (dalloc(c, &c->red, 64) || dalloc(c, &c->red_tmp, 256 * 8) || dalloc(c, &c->d_counters, 8) || dalloc(c, &c->step_idx, 1) ||
         dalloc(c, &c->d_comps, kMaxComp) || ensure_steps(c, 1)) {
         g_create_error = c->err;
         free_list(c->allocs);
@@ -912,10 +936,13 @@ int mdx_cell_ids(mdx_context *c, int64_t n, int64_t *out) {
     if (!c || !out) return MDX_ERR_ARG;
     TRY(need(c, true, true, true));
     if (n != c->n) return fail(c, MDX_ERR_ARG, "n differs from the uploaded count");
-    std::vector<int> tmp((size_t)n);
-    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->ls1id, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    std::vector<int> tmp((size_t)n), orig((size_t)n);
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->mls1id, (size_t)n * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(orig.data(), c->mol[c->cur].orig, (size_t)n * 4,
+                             hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int64_t i = 0; i < n; ++i) out[i] = tmp[i];
+    for (int64_t i = 0; i < n; ++i) out[orig[i]] = tmp[i];
     return MDX_OK;
 }
 

@@ -80,21 +80,19 @@ struct PairParams {
     int nlj;
 };
 
-constexpr int kWavesPerBlock = 4;  // pair kernels: one cell per wavefront
 
 struct PairArgs {
     Geom g;
     SiteArrays s[kTypes];
     PairParams pp;
     int ncell;
-    int nblocks;        // logical blocks (ceil(ncell / kWavesPerBlock))
-    int nblk_per_xcd;   // logical blocks per XCD (launch = 8 * nblk_per_xcd)
+    int nblk_per_xcd;   // cells per XCD (one 64-lane block per cell; launch = 8 * nblk_per_xcd)
     int present[kTypes];
     double *macro_part; // [launch blocks][4]
     unsigned long long *counters;  // [8] or null
 };
 
 typedef void (*PairKernel)(PairArgs);
-PairKernel pair_kernel(int ti, bool fp32, bool count);
+PairKernel pair_kernel(int ti, bool fp32, bool count, bool molsite);
 
 }  // namespace mdx

@@ -210,21 +210,26 @@ __global__ __launch_bounds__(256) void k_leapfrog_post(MolArrays s, const double
     }
 }
 
-// Fixed-order reduction of per-block partials (width values per block) into
-// out[width]: one block, deterministic.
+// Fixed-order reduction of per-block partials (width values per block):
+// block b of the launch sums the contiguous range [b*per, (b+1)*per) of the
+// input blocks into out[b*width + w]. One block with per = nblocks gives the
+// final sum; a many-block pass followed by a one-block pass is the two-level
+// form. Deterministic.
 __global__ __launch_bounds__(256) void k_reduce_partials(const double *__restrict__ partial,
-                                                         int nblocks, int width, double *out) {
+                                                         int nblocks, int width, int per,
+                                                         double *out) {
     __shared__ double red[256];
+    const int b0 = blockIdx.x * per, b1 = min(nblocks, b0 + per);
     for (int w = 0; w < width; ++w) {
         double a = 0;
-        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) a += partial[b * width + w];
+        for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) a += partial[b * width + w];
         red[threadIdx.x] = a;
         __syncthreads();
         for (int o = blockDim.x / 2; o > 0; o >>= 1) {
             if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
             __syncthreads();
         }
-        if (threadIdx.x == 0) out[w] = red[0];
+        if (threadIdx.x == 0) out[blockIdx.x * width + w] = red[0];
         __syncthreads();
     }
 }
@@ -236,18 +241,34 @@ __global__ void k_fill_int(int *a, int n, int v) {
     if (i < n) a[i] = v;
 }
 
-// Cell of every molecule (ls1 index -> inner index) and its arrival rank.
+// Cell of every molecule (ls1 index -> inner index) and its arrival rank in
+// the cell. Lanes of a wavefront that share a cell (molecules are stored in
+// cell order, so a wave touches only a few cells) take one atomic per cell.
 __global__ __launch_bounds__(256) void k_bin_count(MolArrays s, Geom g, int n, int *cell,
                                                    int *rank, int *count, int *ls1id) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double p[3] = {s.rx[i], s.ry[i], s.rz[i]};
-    int ci[3];
-    ls1_cell3(g, p, ci);
-    int c = inner_index(g, ci);
-    cell[i] = c;
-    ls1id[s.orig[i]] = (ci[2] * g.cpd[1] + ci[1]) * g.cpd[0] + ci[0];
-    rank[i] = atomicAdd(&count[c], 1);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool in = i < n;
+    int c = -1;
+    if (in) {
+        double p[3] = {s.rx[i], s.ry[i], s.rz[i]};
+        int ci[3];
+        ls1_cell3(g, p, ci);
+        c = inner_index(g, ci);
+        cell[i] = c;
+        ls1id[i] = (ci[2] * g.cpd[1] + ci[1]) * g.cpd[0] + ci[0];
+    }
+    unsigned long long active = __ballot(in);
+    while (active) {
+        const int leader = __ffsll((long long)active) - 1;
+        const int cl = __shfl(c, leader, 64);
+        const unsigned long long m = __ballot(c == cl) & active;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&count[cl], __popcll(m));
+        base = __shfl(base, leader, 64);
+        if (in && c == cl) rank[i] = base + __popcll(m & ((1ull << lane) - 1ull));
+        active &= ~m;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_bin_scatter(int n, const int *__restrict__ cell,
@@ -265,6 +286,7 @@ __global__ __launch_bounds__(256) void k_bin_gather(int n, MolArrays src, MolArr
                                                     const int *__restrict__ cell,
                                                     const int *__restrict__ perm,
                                                     const int *__restrict__ start, int *newcell,
+                                                    const int *__restrict__ ls1id, int *newls1id,
                                                     const Component *__restrict__ comps,
                                                     int *nsite0, int *nsite1, int *nsite2,
                                                     int *nsite3) {
@@ -284,6 +306,7 @@ __global__ __launch_bounds__(256) void k_bin_gather(int n, MolArrays src, MolArr
     dst.cid[d] = cid;
     dst.orig[d] = src.orig[i];
     newcell[d] = c;
+    newls1id[d] = ls1id[i];
     const Component &cp = comps[cid];
     nsite0[d] = cp.n[0];
     nsite1[d] = cp.n[1];

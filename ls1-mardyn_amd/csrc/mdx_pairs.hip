@@ -24,8 +24,6 @@
 
 namespace mdx {
 
-constexpr int kCap = 48;   // per-lane hit list entries in LDS
-constexpr int kWaves = kWavesPerBlock;
 
 template <typename R> struct V3 { R x, y, z; };
 
@@ -290,58 +288,102 @@ template <> struct Kind<T_D, T_Q> { static constexpr int k = 5; };
 template <> struct Kind<T_Q, T_D> { static constexpr int k = 5; };
 template <> struct Kind<T_Q, T_Q> { static constexpr int k = 6; };
 
-// One site pair seen from site i: force on i, virial (m_i - m_j) x f_i,
-// torque on i (roles swapped where the reference swaps them).
-template <int TI, int TJ, typename R>
-__device__ __forceinline__ void eval_pair(const PairArgs &a, const ISite<R> &I, int j,
-                                          V3<R> off, Acc &acc) {
-    const SiteArrays &J = a.s[TJ];
-    V3<R> pj = SiteView<R>::pos(J, j), mj = SiteView<R>::mol(J, j);
-    pj = {pj.x + off.x, pj.y + off.y, pj.z + off.z};
-    mj = {mj.x + off.x, mj.y + off.y, mj.z + off.z};
-    V3<R> d = {I.pos.x - pj.x, I.pos.y - pj.y, I.pos.z - pj.z};
-    V3<R> dm = {I.mol.x - mj.x, I.mol.y - mj.y, I.mol.z - mj.z};
+// LDS tile: the j sites of the 27 neighbour images of one cell, positions
+// already moved into the i cell's frame (FP64: + image shift, exactly the
+// reference's halo copy r + L; FP32: + offset of the neighbour's corner).
+constexpr int kList = 32;  // per-lane hit list entries
+
+template <int TI, typename R, bool MOLSITE> struct TileCfg {
+    static constexpr bool ORI = TI != T_LJ;   // partners may be dipoles / quadrupoles
+    static constexpr bool MOL = !MOLSITE;     // molecule positions differ from sites
+    static constexpr int CAP = (TI == T_LJ) ? 384 : 256;
+};
+
+template <int TI, typename R, bool MOLSITE> struct Tile {
+    using C = TileCfg<TI, R, MOLSITE>;
+    R px[C::CAP], py[C::CAP], pz[C::CAP];
+    R mx[C::MOL ? C::CAP : 1], my[C::MOL ? C::CAP : 1], mz[C::MOL ? C::CAP : 1];
+    R ex[C::ORI ? C::CAP : 1], ey[C::ORI ? C::CAP : 1], ez[C::ORI ? C::CAP : 1];
+    R par[C::ORI ? C::CAP : 1];
+    int ljid[TI == T_LJ ? C::CAP : 1];
+    unsigned gj[sizeof(R) == 4 ? C::CAP : 1];  // j | k << 27 for the FP64 re-check
+    unsigned short lst[kList][64];
+    int j0[27], pre[28];
+    double shift[27][3];
+};
+
+template <typename R>
+__device__ __forceinline__ V3<R> tile_offset(const Geom &g, const double (*shift)[3], int k);
+template <>
+__device__ __forceinline__ V3<double> tile_offset<double>(const Geom &, const double (*shift)[3],
+                                                          int k) {
+    return {shift[k][0], shift[k][1], shift[k][2]};
+}
+template <>
+__device__ __forceinline__ V3<float> tile_offset<float>(const Geom &g, const double (*)[3], int k) {
+    return {(float)((k % 3 - 1) * g.cl[0]), (float)((k / 3 % 3 - 1) * g.cl[1]),
+            (float)((k / 9 - 1) * g.cl[2])};
+}
+
+// One site pair seen from site i (tile entry s): force on i, virial
+// (m_i - m_j) * f_i, torque on i; roles swapped where the reference swaps.
+template <int TI, int TJ, typename R, bool MOLSITE>
+__device__ __forceinline__ void eval_pair(const PairArgs &a, const ISite<R> &I,
+                                          const Tile<TI, R, MOLSITE> &T, int s, R eps24,
+                                          R sig2, R shift6, Acc &acc) {
+    V3<R> d = {I.pos.x - T.px[s], I.pos.y - T.py[s], I.pos.z - T.pz[s]};
+    V3<R> dm = d;
+    if constexpr (!MOLSITE) dm = {I.mol.x - T.mx[s], I.mol.y - T.my[s], I.mol.z - T.mz[s]};
     PO<R> o;
     o.rf = R(0);
     R sign = R(1);
     V3<R> tq = {R(0), R(0), R(0)};
     if constexpr (TI == T_LJ) {
-        int k = I.ljid * a.pp.nlj + J.ljid[j];
-        if constexpr (sizeof(R) == 8)
-            body_lj<R>(d, a.pp.eps24[k], a.pp.sig2[k], a.pp.shift6[k], o);
-        else
-            body_lj<R>(d, a.pp.feps24[k], a.pp.fsig2[k], a.pp.fshift6[k], o);
-    } else if constexpr (TI == T_C && TJ == T_C) {
-        body_cc<R>(d, I.par, SiteView<R>::par(J, j), o);
-    } else if constexpr (TI == T_C && TJ == T_D) {
-        body_cd<R>(d, I.par, SiteView<R>::ori(J, j), SiteView<R>::par(J, j), o);
-    } else if constexpr (TI == T_C && TJ == T_Q) {
-        body_cq<R>(d, I.par, SiteView<R>::ori(J, j), SiteView<R>::par(J, j), o);
-    } else if constexpr (TI == T_D && TJ == T_C) {
-        V3<R> nd = {-d.x, -d.y, -d.z};
-        body_cd<R>(nd, SiteView<R>::par(J, j), I.e, I.par, o);
-        sign = R(-1);
-        tq = o.M2;
-    } else if constexpr (TI == T_D && TJ == T_D) {
-        body_dd<R>(d, I.e, I.par, SiteView<R>::ori(J, j), SiteView<R>::par(J, j),
-                   (R)a.g.epsRFInvrc3, o);
-        tq = o.M1;
-    } else if constexpr (TI == T_D && TJ == T_Q) {
-        body_dq<R>(d, I.e, I.par, SiteView<R>::ori(J, j), SiteView<R>::par(J, j), o);
-        tq = o.M1;
-    } else if constexpr (TI == T_Q && TJ == T_C) {
-        V3<R> nd = {-d.x, -d.y, -d.z};
-        body_cq<R>(nd, SiteView<R>::par(J, j), I.e, I.par, o);
-        sign = R(-1);
-        tq = o.M2;
-    } else if constexpr (TI == T_Q && TJ == T_D) {
-        V3<R> nd = {-d.x, -d.y, -d.z};
-        body_dq<R>(nd, SiteView<R>::ori(J, j), SiteView<R>::par(J, j), I.e, I.par, o);
-        sign = R(-1);
-        tq = o.M2;
-    } else if constexpr (TI == T_Q && TJ == T_Q) {
-        body_qq<R>(d, I.e, I.par, SiteView<R>::ori(J, j), SiteView<R>::par(J, j), o);
-        tq = o.M1;
+        if (a.pp.nlj > 1) {
+            int k = I.ljid * a.pp.nlj + T.ljid[s];
+            if constexpr (sizeof(R) == 8) {
+                eps24 = a.pp.eps24[k]; sig2 = a.pp.sig2[k]; shift6 = a.pp.shift6[k];
+            } else {
+                eps24 = a.pp.feps24[k]; sig2 = a.pp.fsig2[k]; shift6 = a.pp.fshift6[k];
+            }
+        }
+        body_lj<R>(d, eps24, sig2, shift6, o);
+    }
+    if constexpr (TI != T_LJ) {
+        V3<R> ej = {R(0), R(0), R(0)};
+        if constexpr (TJ >= T_D) ej = {T.ex[s], T.ey[s], T.ez[s]};
+        R pj = T.par[s];
+        if constexpr (TI == T_C && TJ == T_C) {
+            body_cc<R>(d, I.par, pj, o);
+        } else if constexpr (TI == T_C && TJ == T_D) {
+            body_cd<R>(d, I.par, ej, pj, o);
+        } else if constexpr (TI == T_C && TJ == T_Q) {
+            body_cq<R>(d, I.par, ej, pj, o);
+        } else if constexpr (TI == T_D && TJ == T_C) {
+            V3<R> nd = {-d.x, -d.y, -d.z};
+            body_cd<R>(nd, pj, I.e, I.par, o);
+            sign = R(-1);
+            tq = o.M2;
+        } else if constexpr (TI == T_D && TJ == T_D) {
+            body_dd<R>(d, I.e, I.par, ej, pj, (R)a.g.epsRFInvrc3, o);
+            tq = o.M1;
+        } else if constexpr (TI == T_D && TJ == T_Q) {
+            body_dq<R>(d, I.e, I.par, ej, pj, o);
+            tq = o.M1;
+        } else if constexpr (TI == T_Q && TJ == T_C) {
+            V3<R> nd = {-d.x, -d.y, -d.z};
+            body_cq<R>(nd, pj, I.e, I.par, o);
+            sign = R(-1);
+            tq = o.M2;
+        } else if constexpr (TI == T_Q && TJ == T_D) {
+            V3<R> nd = {-d.x, -d.y, -d.z};
+            body_dq<R>(nd, ej, pj, I.e, I.par, o);
+            sign = R(-1);
+            tq = o.M2;
+        } else if constexpr (TI == T_Q && TJ == T_Q) {
+            body_qq<R>(d, I.e, I.par, ej, pj, o);
+            tq = o.M1;
+        }
     }
     V3<R> f = {sign * o.f.x, sign * o.f.y, sign * o.f.z};
     acc.f[0] += (double)f.x;
@@ -361,73 +403,134 @@ __device__ __forceinline__ void eval_pair(const PairArgs &a, const ISite<R> &I, 
     }
 }
 
-template <int TI, int TJ, typename R, bool COUNT>
+// All pairs of the lane's i site with the j sites of type TJ in the 27
+// neighbour images: (1) lanes 0..26 fetch the neighbour ranges in one round,
+// (2) the neighbourhood is staged into LDS in chunks of CAP sites with
+// coalesced, independent loads, (3) stage A tests the molecule-centre cutoff
+// from LDS and keeps per-lane hit lists, (4) stage B evaluates the bodies.
+template <int TI, int TJ, typename R, bool MOLSITE, bool COUNT>
 __device__ __forceinline__ void partner_pass(const PairArgs &a, const ISite<R> &I, bool valid,
-                                             int il_cell, int cx, int cy, int cz, int ph, int P,
-                                             int lane, unsigned (*lst)[64], Acc &acc,
-                                             unsigned long long &ntest) {
-    unsigned long long nhit = 0;
+                                             int cx, int cy, int cz, int ph, int P, int lane,
+                                             Tile<TI, R, MOLSITE> &T, Acc &acc, R eps24, R sig2,
+                                             R shift6, unsigned long long &ntest) {
+    using C = TileCfg<TI, R, MOLSITE>;
     const SiteArrays &J = a.s[TJ];
     const double rc2 = (TI == T_LJ) ? a.g.rclj2 : a.g.rc2;
     const R rc2r = (R)rc2;
-    int cnt = 0, ub = 0;
-    auto drain = [&]() {
-        for (int e = 0;; ++e) {
-            bool act = e < cnt;
-            if (!__any(act)) break;
-            if (act) {
-                unsigned ent = lst[e][lane];
-                int j = (int)(ent & 0x07FFFFFFu);
-                int k = (int)(ent >> 27);
-                Nbr b = neighbour(a.g, cx, cy, cz, k);
-                eval_pair<TI, TJ, R>(a, I, j, image_offset<R>(a.g, b), acc);
-                if (COUNT) ++nhit;
-            }
+    unsigned long long nhit = 0;
+    int nk = 0;
+    if (lane < 27) {
+        Nbr b = neighbour(a.g, cx, cy, cz, lane);
+        int j0 = 0;
+        if (b.ok) {
+            j0 = J.cstart[b.cell];
+            nk = J.cstart[b.cell + 1] - j0;
         }
-        cnt = 0;
-        ub = 0;
-    };
-    for (int k = 0; k < 27; ++k) {
-        Nbr b = neighbour(a.g, cx, cy, cz, k);
-        if (!b.ok) continue;
-        int j0 = J.cstart[b.cell];
-        int nB = J.cstart[b.cell + 1] - j0;
-        if (nB == 0) continue;
-        V3<R> off = image_offset<R>(a.g, b);
-        int per = (nB + P - 1) / P;
+        T.j0[lane] = j0;
+        T.shift[lane][0] = b.s[0];
+        T.shift[lane][1] = b.s[1];
+        T.shift[lane][2] = b.s[2];
+    }
+    int incl = nk;
+    for (int o = 1; o < 32; o <<= 1) {
+        int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane < 27) T.pre[lane + 1] = incl;
+    if (lane == 0) T.pre[0] = 0;
+    const int total = __shfl(incl, 26, 64);
+    __syncthreads();
+    const int own_b = T.pre[13], own_e = T.pre[14];
+    for (int base = 0; base < total; base += C::CAP) {
+        const int nst = min(C::CAP, total - base);
+        for (int s = lane; s < nst; s += 64) {
+            const int idx = base + s;
+            int lo = 0, hi = 27;
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (T.pre[mid] <= idx) lo = mid; else hi = mid;
+            }
+            const int k = lo;
+            const int j = T.j0[k] + (idx - T.pre[k]);
+            const V3<R> off = tile_offset<R>(a.g, T.shift, k);
+            V3<R> p = SiteView<R>::pos(J, j);
+            T.px[s] = p.x + off.x;
+            T.py[s] = p.y + off.y;
+            T.pz[s] = p.z + off.z;
+            if constexpr (C::MOL) {
+                V3<R> m = SiteView<R>::mol(J, j);
+                T.mx[s] = m.x + off.x;
+                T.my[s] = m.y + off.y;
+                T.mz[s] = m.z + off.z;
+            }
+            if constexpr (TI == T_LJ) {
+                T.ljid[s] = J.ljid[j];
+            } else {
+                T.par[s] = SiteView<R>::par(J, j);
+                if constexpr (TJ >= T_D) {
+                    V3<R> e = SiteView<R>::ori(J, j);
+                    T.ex[s] = e.x;
+                    T.ey[s] = e.y;
+                    T.ez[s] = e.z;
+                }
+            }
+            if constexpr (sizeof(R) == 4) T.gj[s] = (unsigned)j | ((unsigned)k << 27);
+        }
+        __syncthreads();
+        // own cell (same molecule exclusion) inside this chunk
+        const int sb = own_b - base, se = own_e - base;
+        int cnt = 0, ub = 0;
+        auto drain = [&]() {
+            for (int e = 0;; ++e) {
+                bool act = e < cnt;
+                if (!__any(act)) break;
+                if (act) {
+                    eval_pair<TI, TJ, R, MOLSITE>(a, I, T, (int)T.lst[e][lane], eps24, sig2,
+                                                  shift6, acc);
+                    if (COUNT) ++nhit;
+                }
+            }
+            cnt = 0;
+            ub = 0;
+        };
+        const int per = (nst + P - 1) / P;
         for (int m = 0; m < per; ++m) {
-            if (ub == kCap) drain();
+            if (ub == kList) drain();
             ++ub;
-            int jj = ph + m * P;
-            bool hit = false;
-            if (valid && jj < nB) {
-                int j = j0 + jj;
-                V3<R> mj = SiteView<R>::mol(J, j);
-                R dx = I.mol.x - (mj.x + off.x), dy = I.mol.y - (mj.y + off.y),
-                  dz = I.mol.z - (mj.z + off.z);
+            const int s = ph + m * P;
+            if (valid && s < nst) {
+                R dx, dy, dz;
+                if constexpr (C::MOL) {
+                    dx = I.mol.x - T.mx[s]; dy = I.mol.y - T.my[s]; dz = I.mol.z - T.mz[s];
+                } else {
+                    dx = I.pos.x - T.px[s]; dy = I.pos.y - T.py[s]; dz = I.pos.z - T.pz[s];
+                }
                 R r2 = dx * dx + dy * dy + dz * dz;
-                hit = r2 < rc2r;
+                bool hit = r2 < rc2r;
+                bool own = s >= sb && s < se;
                 if constexpr (sizeof(R) == 4) {
                     if (fabsf(r2 - rc2r) <= 1e-5f * rc2r) {
-                        double ex = I.mx - (J.mx[j] + b.s[0]);
-                        double ey = I.my - (J.my[j] + b.s[1]);
-                        double ez = I.mz - (J.mz[j] + b.s[2]);
+                        unsigned gj = T.gj[s];
+                        int j = (int)(gj & 0x07FFFFFFu), k = (int)(gj >> 27);
+                        double ex = I.mx - (J.mx[j] + T.shift[k][0]);
+                        double ey = I.my - (J.my[j] + T.shift[k][1]);
+                        double ez = I.mz - (J.mz[j] + T.shift[k][2]);
                         double r2d = ex * ex + ey * ey + ez * ez;
                         hit = r2d < rc2;
-                        if (k == 13 && r2d == 0.0) hit = false;
+                        if (own && r2d == 0.0) hit = false;
                     }
                 }
-                if (k == 13 && r2 == R(0)) hit = false;
+                if (own && r2 == R(0)) hit = false;
                 if (COUNT) ++ntest;
                 if (hit) {
-                    lst[cnt][lane] = (unsigned)j | ((unsigned)k << 27);
+                    T.lst[cnt][lane] = (unsigned short)s;
                     ++cnt;
                 }
             }
         }
+        drain();
+        __syncthreads();
     }
-    drain();
-    (void)il_cell;
     if (COUNT) {
         for (int o = 32; o > 0; o >>= 1) nhit += __shfl_xor(nhit, o, 64);
         if (lane == 0) atomicAdd(&a.counters[1 + Kind<TI, TJ>::k], nhit);
@@ -436,24 +539,30 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const ISite<R> &
 
 __device__ __forceinline__ double shfl_xor_d(double v, int o) { return __shfl_xor(v, o, 64); }
 
-template <int TI, typename R, bool COUNT>
-__global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
-    __shared__ unsigned lst[kWaves][kCap][64];
-    __shared__ double red[kWaves][4];
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-    // XCD-aware block order: hardware block b runs on XCD b % 8; give each XCD
-    // a contiguous range of cells so neighbour data is shared in its L2.
+// One 64-lane block (one wavefront) per cell.
+template <int TI, typename R, bool COUNT, bool MOLSITE>
+__global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
+    __shared__ Tile<TI, R, MOLSITE> T;
+    const int lane = threadIdx.x;
+    // XCD-aware order: hardware block b runs on XCD b % 8; each XCD gets a
+    // contiguous range of cells so neighbour data is shared in its L2.
     const int hb = blockIdx.x;
-    const int lb = (hb & 7) * a.nblk_per_xcd + (hb >> 3);
-    const int cell = lb * kWaves + wv;
+    const int cell = (hb & 7) * a.nblk_per_xcd + (hb >> 3);
     double wmac[3] = {0, 0, 0};
     unsigned long long ntest = 0;
-    if (lb < a.nblocks && cell < a.ncell) {
+    if (cell < a.ncell) {
         const SiteArrays &S = a.s[TI];
         const int i_begin = S.cstart[cell], i_end = S.cstart[cell + 1];
         const int cx = cell % a.g.n[0], cy = (cell / a.g.n[0]) % a.g.n[1],
                   cz = cell / (a.g.n[0] * a.g.n[1]);
+        R eps24 = R(0), sig2 = R(0), shift6 = R(0);
+        if constexpr (TI == T_LJ) {
+            if constexpr (sizeof(R) == 8) {
+                eps24 = a.pp.eps24[0]; sig2 = a.pp.sig2[0]; shift6 = a.pp.shift6[0];
+            } else {
+                eps24 = a.pp.feps24[0]; sig2 = a.pp.fsig2[0]; shift6 = a.pp.fshift6[0];
+            }
+        }
         for (int chunk = i_begin; chunk < i_end; chunk += 64) {
             const int nI = min(64, i_end - chunk);
             int lg = 0;
@@ -464,7 +573,7 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
             const int i = chunk + (valid ? il : 0);
             ISite<R> I;
             I.pos = SiteView<R>::pos(S, i);
-            I.mol = SiteView<R>::mol(S, i);
+            I.mol = MOLSITE ? I.pos : SiteView<R>::mol(S, i);
             I.mx = S.mx[i];
             I.my = S.my[i];
             I.mz = S.mz[i];
@@ -476,22 +585,20 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
             Acc acc;
             for (int k = 0; k < 3; ++k) acc.f[k] = acc.v[k] = acc.m[k] = 0.0;
             acc.u = acc.vir = acc.rf = 0.0;
-            unsigned (*L)[64] = lst[wv];
             if constexpr (TI == T_LJ) {
-                partner_pass<T_LJ, T_LJ, R, COUNT>(a, I, valid, il, cx, cy, cz, ph, P, lane, L, acc,
-                                                  ntest);
+                partner_pass<T_LJ, T_LJ, R, MOLSITE, COUNT>(a, I, valid, cx, cy, cz, ph, P, lane, T,
+                                                            acc, eps24, sig2, shift6, ntest);
             } else {
                 if (a.present[T_C])
-                    partner_pass<TI, T_C, R, COUNT>(a, I, valid, il, cx, cy, cz, ph, P, lane, L,
-                                                    acc, ntest);
+                    partner_pass<TI, T_C, R, MOLSITE, COUNT>(a, I, valid, cx, cy, cz, ph, P, lane, T,
+                                                             acc, eps24, sig2, shift6, ntest);
                 if (a.present[T_D])
-                    partner_pass<TI, T_D, R, COUNT>(a, I, valid, il, cx, cy, cz, ph, P, lane, L,
-                                                    acc, ntest);
+                    partner_pass<TI, T_D, R, MOLSITE, COUNT>(a, I, valid, cx, cy, cz, ph, P, lane, T,
+                                                             acc, eps24, sig2, shift6, ntest);
                 if (a.present[T_Q])
-                    partner_pass<TI, T_Q, R, COUNT>(a, I, valid, il, cx, cy, cz, ph, P, lane, L,
-                                                    acc, ntest);
+                    partner_pass<TI, T_Q, R, MOLSITE, COUNT>(a, I, valid, cx, cy, cz, ph, P, lane, T,
+                                                             acc, eps24, sig2, shift6, ntest);
             }
-            // fold the j phases of each i site
             for (int o = Ipad; o < 64; o <<= 1) {
                 for (int k = 0; k < 3; ++k) {
                     acc.f[k] += shfl_xor_d(acc.f[k], o);
@@ -511,33 +618,21 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
             wmac[2] += acc.rf;
         }
     }
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 3; ++k)
         for (int o = 32; o > 0; o >>= 1) wmac[k] += shfl_xor_d(wmac[k], o);
-    }
-    if (lane == 0) {
-        red[wv][0] = wmac[0];
-        red[wv][1] = wmac[1];
-        red[wv][2] = wmac[2];
-    }
     if (COUNT) {
         for (int o = 32; o > 0; o >>= 1) ntest += __shfl_xor(ntest, o, 64);
         if (lane == 0) atomicAdd(&a.counters[0], ntest);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double u = 0, vir = 0, rf = 0;
-        for (int k = 0; k < kWaves; ++k) {
-            u += red[k][0];
-            vir += red[k][1];
-            rf += red[k][2];
-        }
+    if (lane == 0) {
         double *o = a.macro_part + 4 * hb;
-        o[0] = (TI == T_LJ) ? u : 0.0;
-        o[1] = (TI == T_LJ) ? 0.0 : u;
-        o[2] = vir;
-        o[3] = rf;
+        o[0] = (TI == T_LJ) ? wmac[0] : 0.0;
+        o[1] = (TI == T_LJ) ? 0.0 : wmac[0];
+        o[2] = wmac[1];
+        o[3] = wmac[2];
     }
 }
+
 
 // The cell pairs of the traversal on ls1's halo-extended grid: thread
 // (cell, k) emits (ls1 index of cell, ls1 index of its neighbour k), i.e. the
@@ -556,17 +651,24 @@ __global__ void k_enum_pairs(Geom g, int ncell, int *out) {
 }
 
 // Host-side launcher table.
-PairKernel pair_kernel(int ti, bool fp32, bool count) {
-#define MDX_SEL(T)                                                            \
-    if (ti == T) {                                                            \
-        if (fp32) return count ? k_pairs<T, float, true> : k_pairs<T, float, false>; \
-        return count ? k_pairs<T, double, true> : k_pairs<T, double, false>;  \
+PairKernel pair_kernel(int ti, bool fp32, bool count, bool molsite) {
+#define MDX_SEL2(T, R, C)                                            \
+    return molsite ? k_pairs<T, R, C, true> : k_pairs<T, R, C, false>;
+#define MDX_SEL(T)                                                   \
+    if (ti == T) {                                                   \
+        if (fp32) {                                                  \
+            if (count) { MDX_SEL2(T, float, true) }                  \
+            MDX_SEL2(T, float, false)                                \
+        }                                                            \
+        if (count) { MDX_SEL2(T, double, true) }                     \
+        MDX_SEL2(T, double, false)                                   \
     }
     MDX_SEL(T_LJ)
     MDX_SEL(T_C)
     MDX_SEL(T_D)
     MDX_SEL(T_Q)
 #undef MDX_SEL
+#undef MDX_SEL2
     return nullptr;
 }
 
