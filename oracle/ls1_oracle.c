@@ -40,6 +40,7 @@ typedef struct {
     int nlj, nc, nd, nq;
     double lj[OX_MAXSITE][3], lj_m[OX_MAXSITE], lj_eps[OX_MAXSITE],
         lj_sig[OX_MAXSITE], lj_shift6[OX_MAXSITE];
+    double lj_rcs[OX_MAXSITE]; int lj_shifted[OX_MAXSITE]; /* own shift cutoff, flag */
     double c[OX_MAXSITE][3], c_m[OX_MAXSITE], c_q[OX_MAXSITE];
     double d[OX_MAXSITE][3], d_e[OX_MAXSITE][3], d_p[OX_MAXSITE];
     double qd[OX_MAXSITE][3], q_e[OX_MAXSITE][3], q_Q[OX_MAXSITE];
@@ -111,6 +112,8 @@ static int parse_components(ox_ctx *x, int ncomp, const double *desc, int len) {
             c->lj_eps[s] = v[4];
             c->lj_sig[s] = v[5];
             c->lj_shift6[s] = (v[7] != 0.) ? lj_shift6(v[4], v[5], v[6]) : 0.0;
+            c->lj_rcs[s] = v[6];
+            c->lj_shifted[s] = v[7] != 0.;
             add_mass_site(&c->m, I6, &rotdof, c->lj[s], c->lj_m[s]);
             p += 8;
         }
@@ -267,6 +270,14 @@ void ox_lj_tables(void *h, double *eps24, double *sig2, double *shift6) {
     memcpy(eps24, x->eps24, n * sizeof(double));
     memcpy(sig2, x->sig2, n * sizeof(double));
     memcpy(shift6, x->shift6, n * sizeof(double));
+}
+
+void ox_set_lj_tables(void *h, const double *eps24, const double *sig2, const double *shift6) {
+    ox_ctx *x = (ox_ctx *)h;
+    size_t n = (size_t)x->nlj_total * x->nlj_total;
+    memcpy(x->eps24, eps24, n * sizeof(double));
+    memcpy(x->sig2, sig2, n * sizeof(double));
+    memcpy(x->shift6, shift6, n * sizeof(double));
 }
 
 void ox_component_props(void *h, double *mass, double *inertia, int *rotdof) {
@@ -1075,5 +1086,139 @@ int ox_simulate(void *h, int n, int nsteps, double dt, double *r, double *v, dou
             o[5] = mac[OX_UPOTXPOLES] + mac[OX_MYRF];
         }
     }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ adjoint */
+
+/* How the LJ table entry (a,b) depends on its own (eps_ab, sigma_ab):
+ * shift6 = 24 eps (s6 - s6^2), s6 = (sigma / rcs)^6, when the entry is
+ * shifted with a cutoff that belongs to the pair: different components
+ * (always shifted at rcLJ, Comp2Param.cpp:79-105) or the same site of one
+ * component with do_shift (its own rc_shift, Component.cpp:103-116). Returns
+ * the shift cutoff, or 0 when shift6 does not depend on (eps_ab, sigma_ab)
+ * (unshifted, or two different sites of one component, whose entry carries
+ * site a's own shift: Comp2Param.cpp:27-38). */
+static double pair_shift_rc(const ox_ctx *x, int a, int b) {
+    int ca = -1, cb = -1, sa = 0, sb = 0;
+    for (int c = 0; c < x->ncomp; ++c) {
+        const ox_comp *k = &x->comp[c];
+        if (a >= k->lj_base && a < k->lj_base + k->nlj) { ca = c; sa = a - k->lj_base; }
+        if (b >= k->lj_base && b < k->lj_base + k->nlj) { cb = c; sb = b - k->lj_base; }
+    }
+    if (ca != cb) return x->rcLJ;
+    if (sa == sb && x->comp[ca].lj_shifted[sa]) return x->comp[ca].lj_rcs[sa];
+    return 0.0;
+}
+
+void ox_lj_shift_rc(void *h, double *out) {
+    ox_ctx *x = (ox_ctx *)h;
+    int n = x->nlj_total;
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) out[a * n + b] = pair_shift_rc(x, a, b);
+}
+
+/* Brute-force FP64 adjoint over unordered molecule pairs (minimum image when
+ * periodic), LJ and charge-charge site pairs:
+ *   dU[a*n+b] (and dW) = derivative with respect to the symmetric pair
+ *   parameter eps_ab / sigma_ab, U = sum upot6/6 + sum upotXpoles,
+ *   W = sum (m_i - m_j) . f_site;
+ *   dUq[c*8+s], dWq = derivative with respect to charge s of component c;
+ *   vjp[3i] = sum_j H_ij (cot_i - cot_j), H = d f_site / d r_i (molecule
+ *   translations, orientations held fixed). */
+int ox_adjoint(void *h, int n, const double *r, double *q, const int *cid, const double *cot,
+               double *dUe, double *dUs, double *dWe, double *dWs, double *dUq, double *dWq,
+               double *vjp) {
+    ox_ctx *x = (ox_ctx *)h;
+    const int nl = x->nlj_total;
+    for (int k = 0; k < nl * nl; ++k) dUe[k] = dUs[k] = dWe[k] = dWs[k] = 0;
+    for (int k = 0; k < x->ncomp * OX_MAXSITE; ++k) dUq[k] = dWq[k] = 0;
+    for (int k = 0; k < 3 * n; ++k) vjp[k] = 0;
+    ox_mol *mols = (ox_mol *)malloc(sizeof(ox_mol) * (size_t)(n + 1));
+    if (!mols) return -1;
+    for (int i = 0; i < n; ++i) {
+        setup_mol(x, &mols[i], r + 3 * i, q + 4 * i, cid[i], i);
+        for (int k = 0; k < 4; ++k) q[4 * i + k] = mols[i].q[k];
+    }
+    ox_mol img;
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            ox_mol *B = &mols[j];
+            double sft[3] = {0, 0, 0};
+            if (x->periodic)
+                for (int d = 0; d < 3; ++d) {
+                    double L = x->bbmax[d] - x->bbmin[d], dd = mols[i].r[d] - mols[j].r[d];
+                    if (dd > 0.5 * L) sft[d] = L;
+                    else if (dd < -0.5 * L) sft[d] = -L;
+                }
+            if (sft[0] != 0 || sft[1] != 0 || sft[2] != 0) {
+                double rr[3] = {B->r[0] + sft[0], B->r[1] + sft[1], B->r[2] + sft[2]};
+                setup_mol(x, &img, rr, B->q, B->cid, j);
+                B = &img;
+            }
+            const ox_mol *A = &mols[i];
+            double dm[3] = {A->r[0] - B->r[0], A->r[1] - B->r[1], A->r[2] - B->r[2]};
+            double r2m = dm[0] * dm[0] + dm[1] * dm[1] + dm[2] * dm[2];
+            const ox_comp *ca = &x->comp[A->cid], *cb = &x->comp[B->cid];
+            double dc[3] = {cot[3 * i] - cot[3 * j], cot[3 * i + 1] - cot[3 * j + 1],
+                            cot[3 * i + 2] - cot[3 * j + 2]};
+            if (r2m < x->rcLJ * x->rcLJ) {
+                for (int s = 0; s < ca->nlj; ++s)
+                    for (int t = 0; t < cb->nlj; ++t) {
+                        int a = ca->lj_base + s, b = cb->lj_base + t, k = a * nl + b;
+                        const double *ra = A->s[0][s], *rb = B->s[0][t];
+                        double d[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+                        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2], r2i = 1.0 / r2;
+                        double e24 = x->eps24[k], s2 = x->sig2[k], eps = e24 / 24., sig = sqrt(s2);
+                        double l6 = s2 * r2i; l6 = l6 * l6 * l6;
+                        double l12 = l6 * l6;
+                        double sc = e24 * r2i * (2 * l12 - l6);
+                        double sp = e24 * r2i * r2i * (4 * l6 - 14 * l12);
+                        double dmd = dm[0] * d[0] + dm[1] * d[1] + dm[2] * d[2];
+                        double gU_e = 4 * (l12 - l6), gU_s = 4 * eps * (12 * l12 - 6 * l6) / sig;
+                        double rcs = pair_shift_rc(x, a, b);
+                        if (rcs > 0) {
+                            double s6 = s2 / (rcs * rcs); s6 = s6 * s6 * s6;
+                            gU_e += 4 * (s6 - s6 * s6);
+                            gU_s += 4 * eps * (6 * s6 - 12 * s6 * s6) / sig;
+                        }
+                        double gW_e = dmd * 24 * r2i * (2 * l12 - l6);
+                        double gW_s = dmd * e24 * r2i * (24 * l12 - 6 * l6) / sig;
+                        int kk[2] = {a * nl + b, b * nl + a};
+                        for (int u = 0; u < (a == b ? 1 : 2); ++u) {
+                            dUe[kk[u]] += gU_e; dUs[kk[u]] += gU_s;
+                            dWe[kk[u]] += gW_e; dWs[kk[u]] += gW_s;
+                        }
+                        double ddc = d[0] * dc[0] + d[1] * dc[1] + d[2] * dc[2];
+                        for (int c = 0; c < 3; ++c) {
+                            double g = sc * dc[c] + 2 * sp * d[c] * ddc;
+                            vjp[3 * i + c] += g;
+                            vjp[3 * j + c] -= g;
+                        }
+                    }
+            }
+            if (!(r2m < x->rc * x->rc)) continue;
+            for (int s = 0; s < ca->nc; ++s)
+                for (int t = 0; t < cb->nc; ++t) {
+                    const double *ra = A->s[1][s], *rb = B->s[1][t];
+                    double d[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+                    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2], r2i = 1.0 / r2;
+                    double ri = sqrt(r2i), r3i = r2i * ri;
+                    double qa = ca->c_q[s], qb = cb->c_q[t];
+                    double dmd = dm[0] * d[0] + dm[1] * d[1] + dm[2] * d[2];
+                    dUq[A->cid * OX_MAXSITE + s] += qb * ri;
+                    dUq[B->cid * OX_MAXSITE + t] += qa * ri;
+                    dWq[A->cid * OX_MAXSITE + s] += qb * dmd * r3i;
+                    dWq[B->cid * OX_MAXSITE + t] += qa * dmd * r3i;
+                    double sc = qa * qb * r3i, sp = -1.5 * qa * qb * r3i * r2i;
+                    double ddc = d[0] * dc[0] + d[1] * dc[1] + d[2] * dc[2];
+                    for (int c = 0; c < 3; ++c) {
+                        double g = sc * dc[c] + 2 * sp * d[c] * ddc;
+                        vjp[3 * i + c] += g;
+                        vjp[3 * j + c] -= g;
+                    }
+                }
+        }
+    free(mols);
     return 0;
 }

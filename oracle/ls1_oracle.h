@@ -39,6 +39,7 @@ void ox_destroy(void *h);
 
 int ox_nlj_total(void *h);
 void ox_lj_tables(void *h, double *eps24, double *sig2, double *shift6);
+void ox_set_lj_tables(void *h, const double *eps24, const double *sig2, const double *shift6);
 void ox_component_props(void *h, double *mass, double *inertia, int *rotdof);
 void ox_cell_geometry(void *h, int *cells_per_dim, double *cell_length,
                       double *halo_min, double *halo_max);
@@ -62,6 +63,13 @@ void ox_leapfrog_post(void *h, int n, double dt, double *v, double *L,
 int ox_simulate(void *h, int n, int nsteps, double dt, double *r, double *v,
                 double *q, double *L, const int *cid, double *F, double *M,
                 double *Vi, double *per_step);
+
+/* FP64 adjoint (brute force, LJ and charge-charge): parameter gradients of
+ * Upot and virial and the force vector-Jacobian product; see ls1_oracle.c */
+void ox_lj_shift_rc(void *h, double *out);
+int ox_adjoint(void *h, int n, const double *r, double *q, const int *cid, const double *cot,
+               double *dUe, double *dUs, double *dWe, double *dWs, double *dUq, double *dWq,
+               double *vjp);
 
 /* single site pair, canonical argument order; out = f[3], M1[3], M2[3], u, rf */
 void ox_site_pair(int kind, const double *ri, const double *ei, double pi,

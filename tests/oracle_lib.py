@@ -64,6 +64,10 @@ class Oracle:
                                            _dp, _ip, _dp]
             L.ox_simulate.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp,
                                       _dp, _ip, _dp, _dp, _dp, _dp]
+            L.ox_set_lj_tables.argtypes = [C.c_void_p, _dp, _dp, _dp]
+            L.ox_lj_shift_rc.argtypes = [C.c_void_p, _dp]
+            L.ox_adjoint.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp,
+                                     _dp, _dp, _dp]
             L.ox_site_pair.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp, C.c_double,
                                        C.c_double, C.c_double, C.c_double, C.c_double, _dp]
             cls._lib = L
@@ -105,6 +109,27 @@ class Oracle:
         rot = np.zeros(self.ncomp, np.int32)
         self.lib().ox_component_props(self.h, m, I, rot)
         return m, I.reshape(-1, 3), rot
+
+    def set_lj_tables(self, eps24, sig2, shift6):
+        self.lib().ox_set_lj_tables(self.h, _f64(eps24).ravel(), _f64(sig2).ravel(),
+                                    _f64(shift6).ravel())
+
+    def lj_shift_rc(self):
+        n = self.lib().ox_nlj_total(self.h)
+        out = np.zeros(n * n)
+        self.lib().ox_lj_shift_rc(self.h, out)
+        return out.reshape(n, n)
+
+    def adjoint(self, r, q, cid, cot):
+        n = len(r)
+        nl = self.lib().ox_nlj_total(self.h)
+        dUe, dUs, dWe, dWs = (np.zeros((nl, nl)) for _ in range(4))
+        dUq, dWq = np.zeros((self.ncomp, 8)), np.zeros((self.ncomp, 8))
+        vjp = np.zeros((n, 3))
+        self.lib().ox_adjoint(self.h, n, _f64(r), _f64(q).copy(), np.ascontiguousarray(cid, np.int32),
+                              _f64(cot), dUe, dUs, dWe, dWs, dUq, dWq, vjp)
+        return dict(dU_deps=dUe, dU_dsig=dUs, dW_deps=dWe, dW_dsig=dWs, dU_dq=dUq, dW_dq=dWq,
+                    vjp=vjp)
 
     def cell_geometry(self):
         cpd = np.zeros(3, np.int32)
