@@ -479,7 +479,7 @@ int launch_traverse(mdx_context *c) {
         if (c->nsites[t] == 0) continue;
         a.macro_part = c->macro_part + (size_t)slot * grid * 4;
         PairKernel k = pair_kernel(t, c->precision == MDX_PREC_FP32_FP64ACC, c->counting,
-                                   t == T_LJ && c->lj_at_origin);
+                                   t == T_LJ && c->lj_at_origin, c->model.nlj > 1);
         hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, c->stream, a);
         ++slot;
     }

@@ -93,6 +93,6 @@ struct PairArgs {
 };
 
 typedef void (*PairKernel)(PairArgs);
-PairKernel pair_kernel(int ti, bool fp32, bool count, bool molsite);
+PairKernel pair_kernel(int ti, bool fp32, bool count, bool molsite, bool multi);
 
 }  // namespace mdx
