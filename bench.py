@@ -48,6 +48,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload, scale, tag):
+    """HBM bytes per pair-kernel launch from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated), or
+    None when no summary covers it."""
+    if workload != "ljts_slab" or scale != 1.0 or tag != "fp32":
+        return None
+    f = ROOT / "profiles" / "k_pairs_lj_ljts_slab_3.96M_traffic.json"
+    try:
+        return json.loads(f.read_text())["traffic_bytes"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def make_workload(name, scale):
     if name == "ljts_slab":
         return scenarios.ljts_slab(scale=scale)
@@ -193,7 +206,8 @@ def main():
                    "potential": "LJTS 12-6" if args.workload == "ljts_slab" else args.workload,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak,
+                     "traffic": measured_traffic(args.workload, args.scale, main_tag),
                      "kernel": "k_pairs<LJ>", "flops_per_launch": flops,
                      "kernel_ms": r["pair_ms"]},
     }

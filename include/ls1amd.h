@@ -173,6 +173,42 @@ int mdx_upload_device(mdx_context *ctx, int64_t n, const double *r, const double
 int mdx_set_timing(mdx_context *ctx, int enable);
 int mdx_last_kernel_ms(mdx_context *ctx, double *pair_ms, double *total_ms);
 
+/* The LJ tables in use ([n_ljc][n_ljc] each; any pointer may be NULL):
+ * eps24 = 24 eps_ab, sig2 = sigma_ab^2, shift6, as
+ * VectorizedCellProcessor's _eps_sig / _shift6 (VectorizedCellProcessor.cpp:
+ * 40-80), and shift_rc = the radius at which shift6 is built from the entry's
+ * own (eps, sigma) (Comp2Param.cpp:27-38,79-82,102-105), 0 where shift6 is
+ * zero or borrowed from another site. */
+int mdx_lj_tables(mdx_context *ctx, int n_ljc, double *eps24, double *sig2, double *shift6,
+                  double *shift_rc);
+
+/* Charge of every charge site, q[comp * 8 + site] (replaces the charge
+ * records that Component::addCharge stores, Component.cpp:129-137, and that
+ * FullMolecule::setupSoACache copies into _charges_q, FullMolecule.cpp:
+ * 711-767). Site arrays of an uploaded state are rebuilt at once. A
+ * parameter-fitting caller uses it between adjoint evaluations. */
+int mdx_set_charges(mdx_context *ctx, int ncomp, const double *q);
+
+/* FP64 adjoint of the force path for LJ and charge sites (no counterpart in
+ * the reference, whose force path has no derivatives; it differentiates what
+ * VectorizedCellProcessor::_loopBodyLJ / _loopBodyCharge compute,
+ * VectorizedCellProcessor.cpp:171-277, over the same cell pairs). Needs the
+ * state of the last mdx_update_container. All outputs are optional (NULL).
+ *   dU_deps, dU_dsig, dW_deps, dW_dsig : [nlj][nlj] derivatives of
+ *       U = upot6/6 + upotXpoles and W = virial with respect to the
+ *       symmetric pair parameters eps_ab, sigma_ab of LJ lookup ids a, b
+ *       (eps24 = 24 eps, sig2 = sigma^2); the truncation shift moves with
+ *       them where Comp2Param ties it to them (a site with itself when
+ *       shifted, and every cross-component pair at rcLJ);
+ *   dU_dq, dW_dq : [ncomp][8] derivatives with respect to each charge;
+ *   vjp : [n][3] (upload order) sum_j c_j . dF_j/dr_i for the cotangent
+ *       cot [n][3] on the molecule forces (orientations held fixed).
+ * Models with dipole or quadrupole sites, or more than 16 LJ centres, are
+ * rejected with MDX_ERR_ARG / MDX_ERR_CAPACITY. */
+int mdx_adjoint(mdx_context *ctx, int64_t n, const double *cot, double *dU_deps,
+                double *dU_dsig, double *dW_deps, double *dW_dsig, double *dU_dq,
+                double *dW_dq, double *vjp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,34 @@
+// mdx_adjoint.h -- arguments of the FP64 adjoint kernels (LJ and charge sites).
+#pragma once
+
+#include "mdx_internal.h"
+
+namespace mdx {
+
+constexpr int kAdjMaxLJ = 16;   // LJ lookup ids supported by the adjoint
+constexpr int kAdjBlocks = 4096; // upper bound on adjoint blocks (partials rows)
+
+struct AdjArgs {
+    Geom g;
+    SiteArrays s;              // i and j sites (one type: LJ or charge)
+    PairParams pp;             // FP64 tables (LJ)
+    const double *shift_rc;    // nlj x nlj: radius of the shift tied to (eps, sigma), 0 = none
+    const int *mcid;           // molecule component ids, current order
+    const double *cx, *cy, *cz;  // cotangent on molecule forces, current order (null: no VJP)
+    double *gx, *gy, *gz;      // per-site VJP terms (null: not written)
+    double *part;              // [blocks][K * W] parameter partials
+    int nlj, ncell;
+    int K, W;                  // keys per block table, values per key
+};
+
+typedef void (*AdjKernel)(AdjArgs);
+typedef void (*AdjMolKernel)(int, const int *, const int *, const double *, const double *,
+                             const double *, const double *, const double *, const double *,
+                             double *);
+typedef void (*GatherKernel)(int, const int *, const double *, double *);
+
+AdjKernel adjoint_kernel(int ti);
+AdjMolKernel adjoint_mol_kernel();
+GatherKernel gather_orig3_kernel();
+
+}  // namespace mdx

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/ls1amd.h"
+#include "mdx_adjoint.h"
 #include "mdx_internal.h"
 
 namespace mdx {
@@ -56,11 +57,12 @@ struct mdx_context {
     // model
     Model model{};
     Component *d_comps = nullptr;
-    std::vector<double> site_eps, site_sig, site_shift6, mix;
+    std::vector<double> site_eps, site_sig, site_shift6, site_rcs, mix;
     bool has_mix = false;
     bool lj_at_origin = false;  // every LJ site sits at its molecule's centre
     std::vector<double> eps24, sig2, shift6;
-    double *d_eps24 = nullptr, *d_sig2 = nullptr, *d_shift6 = nullptr;
+    std::vector<double> shift_rc;  // radius of the shift tied to (eps, sigma); 0 = none
+    double *d_eps24 = nullptr, *d_sig2 = nullptr, *d_shift6 = nullptr, *d_shift_rc = nullptr;
     float *d_feps24 = nullptr, *d_fsig2 = nullptr, *d_fshift6 = nullptr;
     double eps_rf = 1.0;
     // domain
@@ -76,6 +78,8 @@ struct mdx_context {
     int *cell = nullptr, *rank = nullptr, *count = nullptr, *start = nullptr, *perm = nullptr,
         *mcell = nullptr, *ls1id = nullptr, *mls1id = nullptr;
     double *red_tmp = nullptr;  // first level of the two-level reductions
+    double *dl_tmp = nullptr;   // download staging (4 n)
+    double *adj_site = nullptr, *adj_cot = nullptr, *adj_part = nullptr, *adj_out = nullptr;
     int *nsite_mol[kTypes] = {nullptr, nullptr, nullptr, nullptr};
     SiteArrays sites[kTypes]{};
     std::vector<int *> scan_tmp;
@@ -158,6 +162,7 @@ int parse_model(mdx_context *c, int ncomp, const double *desc, int len) {
     c->site_eps.clear();
     c->site_sig.clear();
     c->site_shift6.clear();
+    c->site_rcs.clear();
     int p = 0, base = 0;
     for (int ci = 0; ci < ncomp; ++ci) {
         Component &cp = m.comp[ci];
@@ -188,6 +193,7 @@ int parse_model(mdx_context *c, int ncomp, const double *desc, int len) {
             c->site_shift6.push_back(desc[p + 7] != 0. ? lj_shift6(desc[p + 4], desc[p + 5],
                                                                    desc[p + 6])
                                                        : 0.);
+            c->site_rcs.push_back(desc[p + 7] != 0. ? desc[p + 6] : 0.);
             add_mass(desc + p, desc[p + 3]);
         }
         for (int s = 0; s < cp.n[1]; ++s, p += 5) {
@@ -235,6 +241,7 @@ void build_tables(mdx_context *c) {
     c->eps24.assign((size_t)n * n, 0.);
     c->sig2.assign((size_t)n * n, 0.);
     c->shift6.assign((size_t)n * n, 0.);
+    c->shift_rc.assign((size_t)n * n, 0.);
     int k = 0;
     std::vector<double> xi(kMaxComp * kMaxComp, 1.0), eta(kMaxComp * kMaxComp, 1.0);
     for (int i = 0; i < m.ncomp; ++i)
@@ -253,6 +260,9 @@ void build_tables(mdx_context *c) {
                 c->eps24[(size_t)ii * n + jj] = 24. * sqrt(eps[ii] * eps[jj]);
                 c->sig2[(size_t)ii * n + jj] = s2;
                 c->shift6[(size_t)ii * n + jj] = c->site_shift6[ii];
+                // only a site's own entry has its shift built from the pair's
+                // (eps, sigma); other same-component entries borrow site ii's
+                if (si == sj) c->shift_rc[(size_t)ii * n + jj] = c->site_rcs[ii];
             }
         for (int cj = ci + 1; cj < m.ncomp; ++cj) {
             const Component &b = m.comp[cj];
@@ -270,6 +280,7 @@ void build_tables(mdx_context *c) {
                         c->eps24[(size_t)ii * n + jj] = e24;
                         c->sig2[(size_t)ii * n + jj] = s2;
                         c->shift6[(size_t)ii * n + jj] = e24 * (sr6 - sr6 * sr6);
+                        c->shift_rc[(size_t)ii * n + jj] = c->rc_lj;
                     }
             }
         }
@@ -288,6 +299,8 @@ int upload_tables(mdx_context *c) {
     HIPCHK(c, hipMemcpy(c->d_eps24, c->eps24.data(), nn * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_sig2, c->sig2.data(), nn * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_shift6, c->shift6.data(), nn * 8, hipMemcpyHostToDevice));
+    if (c->shift_rc.size() == nn)
+        HIPCHK(c, hipMemcpy(c->d_shift_rc, c->shift_rc.data(), nn * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_feps24, fe.data(), nn * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_fsig2, fs.data(), nn * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_fshift6, fh.data(), nn * 4, hipMemcpyHostToDevice));
@@ -578,6 +591,8 @@ int state_setup(mdx_context *c, int64_t n64, const int32_t *cid_host) {
     int grid = pair_grid(c, &px);
     TRY(dalloc(c, &c->macro_part, (size_t)grid * 4 * kTypes, true));
     TRY(dalloc(c, &c->lf_part, (size_t)nblk(n) * 4, true));
+    TRY(dalloc(c, &c->dl_tmp, (size_t)n * 4, true));
+    c->adj_site = c->adj_cot = c->adj_part = c->adj_out = nullptr;  // made on first use
     HIPCHK(c, hipMemsetAsync(c->F, 0, 3 * (size_t)n * 8, c->stream));
     HIPCHK(c, hipMemsetAsync(c->M, 0, 3 * (size_t)n * 8, c->stream));
     HIPCHK(c, hipMemsetAsync(c->Vi, 0, 3 * (size_t)n * 8, c->stream));
@@ -656,6 +671,7 @@ int mdx_set_components(mdx_context *c, int ncomp, const double *desc, int desc_l
     TRY(dalloc(c, &c->d_eps24, nn));
     TRY(dalloc(c, &c->d_sig2, nn));
     TRY(dalloc(c, &c->d_shift6, nn));
+    TRY(dalloc(c, &c->d_shift_rc, nn));
     TRY(dalloc(c, &c->d_feps24, nn));
     TRY(dalloc(c, &c->d_fsig2, nn));
     TRY(dalloc(c, &c->d_fshift6, nn));
@@ -776,8 +792,7 @@ int mdx_download(mdx_context *c, int64_t n, double *r, double *v, double *q, dou
     HIPCHK(c, hipSetDevice(c->device));
     const int nn = c->n;
     MolArrays &m = c->mol[c->cur];
-    double *tmp;
-    TRY(dalloc(c, &tmp, (size_t)nn * 4, true));
+    double *tmp = c->dl_tmp;
     std::vector<double> hb((size_t)nn * 4);
     auto get = [&](const double *a, int width, int stride_in, double *out) -> int {
         if (!out) return MDX_OK;
@@ -997,6 +1012,136 @@ int mdx_cell_pairs(mdx_context *c, int32_t *pairs, int64_t max_pairs, int64_t *c
             pairs[2 * k + 1] = b;
             ++k;
         }
+    }
+    return MDX_OK;
+}
+
+int mdx_lj_tables(mdx_context *c, int n_ljc, double *eps24, double *sig2, double *shift6,
+                  double *shift_rc) {
+    if (!c) return MDX_ERR_ARG;
+    TRY(need(c, true, true, false));
+    if (n_ljc != c->model.nlj) return fail(c, MDX_ERR_ARG, "n_ljc differs from the model");
+    const size_t nn = (size_t)n_ljc * n_ljc;
+    if (eps24) std::copy(c->eps24.begin(), c->eps24.begin() + nn, eps24);
+    if (sig2) std::copy(c->sig2.begin(), c->sig2.begin() + nn, sig2);
+    if (shift6) std::copy(c->shift6.begin(), c->shift6.begin() + nn, shift6);
+    if (shift_rc) std::copy(c->shift_rc.begin(), c->shift_rc.begin() + nn, shift_rc);
+    return MDX_OK;
+}
+
+int mdx_set_charges(mdx_context *c, int ncomp, const double *q) {
+    if (!c || !q) return MDX_ERR_ARG;
+    TRY(need(c, true, false, false));
+    if (ncomp != c->model.ncomp) return fail(c, MDX_ERR_ARG, "ncomp differs from the model");
+    for (int k = 0; k < ncomp; ++k)
+        for (int s = 0; s < c->model.comp[k].n[T_C]; ++s)
+            c->model.comp[k].param[T_C][s] = q[k * kMaxSite + s];
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->d_comps, c->model.comp, sizeof(Component) * ncomp,
+                             hipMemcpyHostToDevice, c->stream));
+    // the site arrays carry the charges: rebuild them for the current state
+    // (re-binning an unchanged state keeps its order)
+    if (c->have_state) TRY(launch_update(c));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MDX_OK;
+}
+
+int mdx_adjoint(mdx_context *c, int64_t n, const double *cot, double *dU_deps, double *dU_dsig,
+                double *dW_deps, double *dW_dsig, double *dU_dq, double *dW_dq, double *vjp) {
+    if (!c) return MDX_ERR_ARG;
+    TRY(need(c, true, true, true));
+    if (n != c->n) return fail(c, MDX_ERR_ARG, "n differs from the uploaded count");
+    if (vjp && !cot) return fail(c, MDX_ERR_ARG, "vjp requested without a cotangent");
+    if (c->nsites[T_D] > 0 || c->nsites[T_Q] > 0)
+        return fail(c, MDX_ERR_ARG, "the adjoint covers LJ and charge sites only");
+    const int nl = c->model.nlj;
+    if (nl > kAdjMaxLJ) return fail(c, MDX_ERR_CAPACITY, "the adjoint supports <= 16 LJ centres");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nn = c->n, ncomp = c->model.ncomp;
+    const int nsl = c->nsites[T_LJ], nsc = c->nsites[T_C];
+    if (!c->adj_site) {
+        TRY(dalloc(c, &c->adj_site, 3 * ((size_t)nsl + nsc), true));
+        TRY(dalloc(c, &c->adj_cot, 3 * (size_t)nn, true));
+        TRY(dalloc(c, &c->adj_part, (size_t)kAdjBlocks * kAdjMaxLJ * kAdjMaxLJ * 4, true));
+        TRY(dalloc(c, &c->adj_out, (size_t)kAdjMaxLJ * kAdjMaxLJ * 4 + kMaxComp * kMaxSite * 2,
+                   true));
+    }
+    MolArrays &m = c->mol[c->cur];
+    if (cot) {
+        HIPCHK(c, hipMemcpyAsync(c->dl_tmp, cot, (size_t)nn * 3 * 8, hipMemcpyHostToDevice,
+                                 c->stream));
+        hipLaunchKernelGGL(gather_orig3_kernel(), dim3(nblk(nn)), dim3(kBlock), 0, c->stream, nn,
+                           m.orig, c->dl_tmp, c->adj_cot);
+    }
+    const int grid = std::max(1, std::min(c->ncell, kAdjBlocks));
+    double *out_lj = c->adj_out, *out_q = c->adj_out + (size_t)kAdjMaxLJ * kAdjMaxLJ * 4;
+    double *g_site[2] = {c->adj_site, c->adj_site + 3 * (size_t)nsl};
+    const int ts[2] = {T_LJ, T_C};
+    for (int u = 0; u < 2; ++u) {
+        const int t = ts[u];
+        if (c->nsites[t] == 0) continue;
+        AdjArgs a;
+        memset(&a, 0, sizeof(a));
+        a.g = c->g;
+        a.s = c->sites[t];
+        a.pp.eps24 = c->d_eps24;
+        a.pp.sig2 = c->d_sig2;
+        a.pp.shift6 = c->d_shift6;
+        a.pp.nlj = nl;
+        a.shift_rc = c->d_shift_rc;
+        a.mcid = m.cid;
+        if (cot) {
+            a.cx = c->adj_cot;
+            a.cy = c->adj_cot + nn;
+            a.cz = c->adj_cot + 2 * (size_t)nn;
+            const size_t ns = c->nsites[t];
+            a.gx = g_site[u];
+            a.gy = g_site[u] + ns;
+            a.gz = g_site[u] + 2 * ns;
+        }
+        a.part = c->adj_part;
+        a.nlj = nl;
+        a.ncell = c->ncell;
+        a.K = t == T_LJ ? nl : ncomp * kMaxSite;
+        a.W = t == T_LJ ? 4 * nl : 2;
+        const size_t lds = ((size_t)a.W * 64 + (size_t)a.K * a.W) * 8 + 64 * 4;
+        hipLaunchKernelGGL(adjoint_kernel(t), dim3(grid), dim3(64), lds, c->stream, a);
+        const int width = a.K * a.W;
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->stream, c->adj_part, grid,
+                           width, grid, t == T_LJ ? out_lj : out_q);
+    }
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> hl((size_t)nl * nl * 4, 0.0), hq((size_t)ncomp * kMaxSite * 2, 0.0);
+    if (nsl) HIPCHK(c, hipMemcpyAsync(hl.data(), out_lj, hl.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    if (nsc) HIPCHK(c, hipMemcpyAsync(hq.data(), out_q, hq.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    if (vjp) {
+        const size_t s0 = nsl, s1 = nsc;
+        hipLaunchKernelGGL(adjoint_mol_kernel(), dim3(nblk(nn)), dim3(kBlock), 0, c->stream, nn,
+                           c->sites[T_LJ].mstart, c->sites[T_C].mstart,
+                           nsl ? g_site[0] : nullptr, g_site[0] + s0, g_site[0] + 2 * s0,
+                           nsc ? g_site[1] : nullptr, g_site[1] + s1, g_site[1] + 2 * s1,
+                           c->dl_tmp);
+        hipLaunchKernelGGL(k_scatter_orig, dim3(nblk(nn)), dim3(kBlock), 0, c->stream, nn, m.orig,
+                           c->dl_tmp, c->adj_cot, nn, 3);
+        HIPCHK(c, hipMemcpyAsync(vjp, c->adj_cot, (size_t)nn * 3 * 8, hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // each visit holds half of its pair's term under (a, b) = (own id, other
+    // id): the symmetric parameter's derivative sums both orders
+    double *outs[4] = {dU_deps, dU_dsig, dW_deps, dW_dsig};
+    for (int g = 0; g < 4; ++g) {
+        if (!outs[g]) continue;
+        for (int x = 0; x < nl; ++x)
+            for (int y = 0; y < nl; ++y) {
+                double v = hl[((size_t)x * nl + y) * 4 + g];
+                if (x != y) v += hl[((size_t)y * nl + x) * 4 + g];
+                outs[g][(size_t)x * nl + y] = v;
+            }
+    }
+    for (int k = 0; k < ncomp * kMaxSite; ++k) {
+        if (dU_dq) dU_dq[k] = hq[2 * k];
+        if (dW_dq) dW_dq[k] = hq[2 * k + 1];
     }
     return MDX_OK;
 }

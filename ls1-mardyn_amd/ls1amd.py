@@ -28,7 +28,8 @@ EXPORTS = (
     "mdx_set_domain", "mdx_set_precision", "mdx_upload", "mdx_download", "mdx_update_container",
     "mdx_traverse", "mdx_leapfrog_pre", "mdx_leapfrog_post", "mdx_step", "mdx_synchronize",
     "mdx_cell_ids", "mdx_cell_geometry", "mdx_cell_pairs", "mdx_set_counting", "mdx_counters",
-    "mdx_upload_device", "mdx_set_timing", "mdx_last_kernel_ms",
+    "mdx_upload_device", "mdx_set_timing", "mdx_last_kernel_ms", "mdx_set_charges",
+    "mdx_adjoint", "mdx_lj_tables",
 )
 
 _lib = None
@@ -70,6 +71,9 @@ def load():
     L.mdx_upload_device.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp]
     L.mdx_set_timing.argtypes = [vp, C.c_int]
     L.mdx_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.mdx_set_charges.argtypes = [vp, C.c_int, _dp]
+    L.mdx_lj_tables.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+    L.mdx_adjoint.argtypes = [vp, C.c_int64] + [vp] * 8
     _lib = L
     return L
 
@@ -115,6 +119,11 @@ class Context:
     def setup(self, ncomp, comp_desc, mix, eps_rf, box, rc, rc_lj, periodic=True, cell_mode=0,
               precision=PREC_FP64):
         desc = np.ascontiguousarray(comp_desc, np.float64)
+        self.ncomp, self.nlj, pos = int(ncomp), 0, 0
+        for _ in range(self.ncomp):  # flat component records: counts, sites, moments
+            nlj, nc, nd, nq = (int(x) for x in desc[pos:pos + 4])
+            self.nlj += nlj
+            pos += 4 + 8 * nlj + 5 * nc + 7 * nd + 7 * nq + 3
         self._mix = np.ascontiguousarray(mix, np.float64).ravel() if mix is not None and len(mix) else None
         self._chk(self.L.mdx_set_components(self.h, int(ncomp), desc, desc.size, _ptr(self._mix),
                                             float(eps_rf)), "mdx_set_components")
@@ -228,6 +237,36 @@ class Context:
         a, b = C.c_double(0), C.c_double(0)
         self._chk(self.L.mdx_last_kernel_ms(self.h, C.byref(a), C.byref(b)), "mdx_last_kernel_ms")
         return a.value, b.value
+
+
+    def lj_tables(self):
+        """(eps24, sig2, shift6, shift_rc), each [nlj, nlj]."""
+        nl = self.nlj
+        t = [np.zeros((nl, nl)) for _ in range(4)]
+        self._chk(self.L.mdx_lj_tables(self.h, nl, *(_ptr(x) for x in t)), "mdx_lj_tables")
+        return tuple(t)
+
+    def set_charges(self, q):
+        """q[comp, site] (ncomp x 8) charges; rebuilds the site arrays."""
+        q = np.ascontiguousarray(q, np.float64)
+        self._chk(self.L.mdx_set_charges(self.h, q.shape[0], q.ravel()), "mdx_set_charges")
+
+    def adjoint(self, cot=None):
+        """FP64 adjoint for the state of the last update_container:
+        dU/dW with respect to LJ pair eps/sigma ([nlj, nlj]) and charges
+        ([ncomp, 8]); with a cotangent on the molecule forces, the force
+        vector-Jacobian product with respect to positions ([n, 3])."""
+        n, nl, ncomp = self.n, self.nlj, self.ncomp
+        out = {k: np.zeros((nl, nl)) for k in ("dU_deps", "dU_dsig", "dW_deps", "dW_dsig")}
+        out["dU_dq"], out["dW_dq"] = np.zeros((ncomp, 8)), np.zeros((ncomp, 8))
+        c = None
+        if cot is not None:
+            c = np.ascontiguousarray(cot, np.float64).reshape(n, 3)
+            out["vjp"] = np.zeros((n, 3))
+        self._chk(self.L.mdx_adjoint(self.h, n, _ptr(c), *(_ptr(out[k]) for k in (
+            "dU_deps", "dU_dsig", "dW_deps", "dW_dsig", "dU_dq", "dW_dq")), _ptr(out.get("vjp"))),
+            "mdx_adjoint")
+        return out
 
 
 def upot_virial(macro):
