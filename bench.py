@@ -136,6 +136,167 @@ def cpu_baseline(workload, budget_s):
                 sample=f"{sample}, {nsteps} steps, {dt:.1f} s")
 
 
+def run_replicas(args):
+    """Batched replicas: all replicas in one context, one launch per kernel
+    per step over the flattened cell table, one hipGraph per step, autotuned
+    pair-kernel launch shape; compared with the default launch shape and with
+    per-replica plain launches (one context and graph per replica, stepped
+    one after another)."""
+    reps = scenarios.replicas(nrep=args.replicas)
+    ntot = int(sum(s.n for s in reps))
+    dt = reps[0].dt
+    log(f"replicas: {len(reps)} systems, {ntot} molecules")
+    out = {}
+    with Context(0) as ctx:
+        ctx.setup_replicas(reps, periodic=True, precision=PREC_FP32_FP64ACC)
+        ctx.update_container()
+        ctx.traverse()
+        ctx.set_counting(True)
+        ctx.traverse()
+        counters = ctx.counters()
+        ctx.set_counting(False)
+        timed = {}
+        for tag in ("default", "autotuned"):
+            if tag == "autotuned":
+                (cpb, bpx), best = ctx.autotune(reps=3)
+                out["autotuned_shape"] = {"cells_per_block": cpb, "blocks_per_xcd": bpx}
+            ctx.set_timing(True)
+            pm = []
+            for _ in range(args.kernel_reps):
+                ctx.traverse()
+                pm.append(ctx.last_kernel_ms()[0])
+            ctx.set_timing(False)
+            ctx.step(args.warmup, dt, use_graph=True)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.step(args.steps, dt, use_graph=True)
+            ctx.synchronize()
+            sec = time.perf_counter() - t0
+            timed[tag] = {"value": ntot * args.steps / sec, "ms_per_step": sec * 1e3 / args.steps,
+                          "pair_kernel_ms": float(np.mean(pm))}
+            log(f"  {tag}: {timed[tag]['ms_per_step']:.3f} ms/step, pair {timed[tag]['pair_kernel_ms']:.3f} ms")
+    out.update(timed)
+    # per-replica plain launches
+    ctxs = []
+    try:
+        for s in reps:
+            c = Context(0)
+            ctxs.append(c)
+            c.setup_system(s, periodic=True, precision=PREC_FP32_FP64ACC)
+            c.update_container()
+            c.traverse()
+            c.step(1, dt, use_graph=True)
+        for c in ctxs:
+            c.synchronize()
+        t0 = time.perf_counter()
+        for c in ctxs:
+            c.step(args.plain_steps, dt, use_graph=True)
+        for c in ctxs:
+            c.synchronize()
+        sec = time.perf_counter() - t0
+        out["per_replica_launches"] = {"value": ntot * args.plain_steps / sec,
+                                       "ms_per_step": sec * 1e3 / args.plain_steps,
+                                       "steps": args.plain_steps}
+        log(f"  per-replica launches: {out['per_replica_launches']['ms_per_step']:.3f} ms/step")
+    finally:
+        for c in ctxs:
+            c.close()
+    flops = float((FLOPS_PER_PAIR * counters).sum())
+    best = out["autotuned"]
+    achieved = flops / (best["pair_kernel_ms"] * 1e-3) / 1e12
+    line = {
+        "metric": "molecule-steps/s on 1 MI355X, batched replicas (FP32 pair / FP64 accumulate)",
+        "value": best["value"], "unit": "molecule-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": best["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pair arithmetic, f64 accumulation/sums/Leapfrog",
+        "data": "synthetic (seeded random sequential addition, Maxwell-Boltzmann velocities)",
+        "config": {"workload": "replicas", "replicas": len(reps), "molecules": ntot,
+                   "n_range": [2000, 40000], "rho_range": [0.05, 0.85], "T_range": [0.7, 1.3],
+                   "rc": 2.5, "dt": dt, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VECTOR_TF,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VECTOR_TF, "traffic": None,
+                     "kernel": "k_pairs<LJ>", "flops_per_launch": flops,
+                     "kernel_ms": best["pair_kernel_ms"]},
+        "replicas_detail": out,
+        "speedup_vs_per_replica_launches": best["value"] / out["per_replica_launches"]["value"],
+    }
+    if not args.no_cpu_baseline:
+        from oracle_lib import Oracle
+
+        t0, done, k = time.perf_counter(), 0, 0
+        while time.perf_counter() - t0 < args.cpu_budget and k < len(reps):
+            s = reps[k]
+            o = Oracle(s.ncomp, s.comp_desc, s.mix, s.eps_rf, s.rc, s.rc_lj, s.box, periodic=True)
+            o.simulate(2, s.dt, s.r, s.v, s.q, s.L, s.cid)
+            done += 2 * s.n
+            k += 1
+        sec = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": done / sec, "unit": "molecule-steps/s", "cores": 1,
+                                "kind": "port",
+                                "sample": f"first {k} replicas run one after another, 2 steps each, {sec:.1f} s"}
+    return line
+
+
+def rel_rms(a, b):
+    den = float((b ** 2).sum(1).mean())
+    return float(np.sqrt(((a - b) ** 2).sum(1).mean() / den)) if den > 0 else None
+
+
+def precision_report(scale, drift_steps):
+    """Error of the FP32-pair / FP64-accumulate form against the plain FP64
+    form (which matches the FP64 oracle to 1e-12, tests/test_gpu_parity.py):
+    one traversal of the multi-site mixture and of the LJTS slab (force,
+    torque, Upot, virial), and total-energy drift over drift_steps Leapfrog
+    steps of the 32,000-molecule FCC LJ fluid (full LJ as specified, and the
+    truncated-shifted variant whose energy is continuous at rc)."""
+    from ls1amd import upot_virial
+
+    out = {}
+    cases = (("mixture", lambda: scenarios.mixture(n=int(1_000_000 * scale ** 3))),
+             ("ljts_slab", lambda: scenarios.ljts_slab(scale=scale)))
+    for name, make in cases:
+        s = make()
+        log(f"precision: {name} N = {s.n}")
+        res = {}
+        for tag, p in (("fp32", PREC_FP32_FP64ACC), ("fp64", PREC_FP64)):
+            with Context(0) as ctx:
+                ctx.setup_system(s, periodic=True, precision=p)
+                ctx.update_container()
+                mac = ctx.traverse()
+                d = ctx.download()
+                res[tag] = (d, upot_virial(mac))
+        (d32, (u32, w32)), (d64, (u64, w64)) = res["fp32"], res["fp64"]
+        out[name] = {"molecules": int(s.n),
+                     "force_rel_rms": rel_rms(d32["F"], d64["F"]),
+                     "force_max_abs_over_rms": float(np.abs(d32["F"] - d64["F"]).max()
+                                                     / np.sqrt((d64["F"] ** 2).sum(1).mean())),
+                     "torque_rel_rms": rel_rms(d32["M"], d64["M"]),
+                     "virial_tensor_rel_rms": rel_rms(d32["Vi"], d64["Vi"]),
+                     "upot_rel": abs(u32 - u64) / abs(u64), "virial_rel": abs(w32 - w64) / abs(w64)}
+    for shifted in (False, True):
+        s = scenarios.lj_fcc(shifted=shifted)
+        key = f"energy_drift_lj_fcc_{'shifted' if shifted else 'full'}_{drift_steps}_steps"
+        log(f"precision: {key}")
+        rec = {}
+        for tag, p in (("fp32", PREC_FP32_FP64ACC), ("fp64", PREC_FP64)):
+            with Context(0) as ctx:
+                ctx.setup_system(s, periodic=True, precision=p)
+                ctx.update_container()
+                u0, _ = upot_virial(ctx.traverse())
+                e0 = u0 + 0.5 * float((s.v ** 2).sum())  # m = 1
+                per = np.concatenate([ctx.step(min(1000, drift_steps - k), s.dt, use_graph=True)
+                                      for k in range(0, drift_steps, 1000)])
+            e = per[:, 0] + 0.5 * (per[:, 2] + per[:, 3])
+            rec[tag] = {"E0": e0, "E_end": float(e[-1]), "drift_rel": float((e[-1] - e0) / abs(e0)),
+                        "max_dev_rel": float(np.abs(e - e0).max() / abs(e0)),
+                        "upot_end": float(per[-1, 0])}
+        rec["energy_diff_end_rel"] = abs(rec["fp32"]["E_end"] - rec["fp64"]["E_end"]) / abs(
+            rec["fp64"]["E_end"])
+        out[key] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,7 +311,18 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--precision-report", action="store_true",
+                    help="only report the FP32 form's errors against FP64 (one JSON line)")
+    ap.add_argument("--drift-steps", type=int, default=10_000)
+    ap.add_argument("--replicas", type=int, default=512,
+                    help="replica count of the batched-replicas workload")
+    ap.add_argument("--plain-steps", type=int, default=10,
+                    help="steps of the per-replica plain-launch comparison")
     args = ap.parse_args()
+    if args.precision_report:
+        print(json.dumps({"precision_report": precision_report(args.scale, args.drift_steps)}),
+              flush=True)
+        return
     if args.calibrate_copy:
         import torch
 
@@ -166,6 +338,10 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "replicas":
+        print(json.dumps(run_replicas(args)), flush=True)
+        return
 
     log(f"generating workload {args.workload} scale {args.scale}")
     s = make_workload(args.workload, args.scale)

@@ -91,6 +91,32 @@ int mdx_set_lj_tables(mdx_context *ctx, int n_ljc, const double *eps24, const do
 int mdx_set_domain(mdx_context *ctx, const double *box, double rc, double rc_lj, int periodic,
                    int cell_mode);
 
+/* Batched replicas: nrep independent systems that share the component model,
+ * the cutoffs and the boundary mode, each in its own box [0, boxes[3r..3r+2])
+ * with its own linked cells (the LinkedCells rules of mdx_set_domain per
+ * replica), stored back to back in one context and stepped together: every
+ * kernel of a step is one launch over all replicas (the pair kernel walks the
+ * flattened cell table). counts[r] molecules belong to replica r; mdx_upload
+ * then takes sum(counts) molecules, replica by replica. Per-replica outputs:
+ * macro [nrep][4] (mdx_traverse), sums [nrep][4] (mdx_leapfrog_post),
+ * per_step [nsteps][nrep][6] (mdx_step). No counterpart in the reference,
+ * which runs one system per process; each replica is what one ls1 run with
+ * the same input computes. mdx_cell_ids / mdx_cell_geometry / mdx_cell_pairs
+ * / mdx_adjoint serve single systems only. */
+int mdx_set_replicas(mdx_context *ctx, int nrep, const double *boxes, const int64_t *counts,
+                     double rc, double rc_lj, int periodic, int cell_mode);
+
+/* Launch shape of the pair kernels: cells_per_block consecutive cells per
+ * work item (1..64), and blocks_per_xcd persistent 64-lane blocks per XCD
+ * walking the items (0: one block per item). Default (1, 0). */
+int mdx_set_pair_config(mdx_context *ctx, int cells_per_block, int blocks_per_xcd);
+
+/* Times the pair kernels of the current state over the candidate launch
+ * shapes (cells_per_block 1, 2, 4, 8 x blocks_per_xcd 0, 64, 128, 256, 512),
+ * reps timed traversals each (HIP events), and keeps the fastest. chosen
+ * (may be NULL) = {cells_per_block, blocks_per_xcd}; best_ms = its time. */
+int mdx_autotune(mdx_context *ctx, int reps, int32_t *chosen, double *best_ms);
+
 /* MDX_PREC_FP64 (default) or MDX_PREC_FP32_FP64ACC. */
 int mdx_set_precision(mdx_context *ctx, int precision);
 

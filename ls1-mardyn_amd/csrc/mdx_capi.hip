@@ -13,13 +13,15 @@
 #include "mdx_internal.h"
 
 namespace mdx {
-__global__ void k_leapfrog_pre(MolArrays, const double *, const double *, const Component *, Geom,
+__global__ void k_leapfrog_pre(MolArrays, const double *, const double *, const Component *, Reps,
                                double, int);
 __global__ void k_leapfrog_post(MolArrays, const double *, const double *, const Component *,
-                                double, int, double *);
+                                double, int, const int *, const int *, double *);
+__global__ void k_seg_reduce(const double *, int, const int *, double *);
 __global__ void k_reduce_partials(const double *, int, int, int, double *);
 __global__ void k_fill_int(int *, int, int);
-__global__ void k_bin_count(MolArrays, Geom, int, int *, int *, int *, int *);
+__global__ void k_fill_double(double *, int, double);
+__global__ void k_bin_count(MolArrays, Reps, int, int *, int *, int *, int *);
 __global__ void k_bin_scatter(int, const int *, const int *, const int *, int *);
 __global__ void k_bin_gather(int, MolArrays, MolArrays, const int *, const int *, const int *,
                              int *, const int *, int *, const Component *, int *, int *, int *,
@@ -28,13 +30,13 @@ __global__ void k_scan_block(const int *, int *, int, int *);
 __global__ void k_scan_add(int *, int, const int *);
 __global__ void k_scan_total(const int *, const int *, int, int *);
 __global__ void k_cell_site_start(int, const int *, const int *, int *);
-__global__ void k_build_sites(int, MolArrays, Geom, const int *, const Component *, SiteArrays,
+__global__ void k_build_sites(int, MolArrays, Reps, const int *, const Component *, SiteArrays,
                               SiteArrays, SiteArrays, SiteArrays);
 __global__ void k_calc_fm(int, MolArrays, const Component *, SiteArrays, SiteArrays, SiteArrays,
                           SiteArrays, double *, double *, double *);
 __global__ void k_scatter_orig(int, const int *, const double *, double *, int, int);
-__global__ void k_finalize_macro(const double *, double *, double *, const int *);
-__global__ void k_finalize_lf(const double *, double *, const int *);
+__global__ void k_finalize_macro(const double *, double *, double *, const int *, int);
+__global__ void k_finalize_lf(const double *, double *, const int *, int);
 __global__ void k_step_advance(int *);
 __global__ void k_enum_pairs(Geom, int, int *);
 }  // namespace mdx
@@ -65,10 +67,22 @@ struct mdx_context {
     double *d_eps24 = nullptr, *d_sig2 = nullptr, *d_shift6 = nullptr, *d_shift_rc = nullptr;
     float *d_feps24 = nullptr, *d_fsig2 = nullptr, *d_fshift6 = nullptr;
     double eps_rf = 1.0;
-    // domain
+    // domain: nrep replicas (1 for a single system; g = replica 0)
     Geom g{};
     double rc = 0, rc_lj = 0;
-    int ncell = 0;
+    int ncell = 0;      // global cells over all replicas
+    int nrep = 1;
+    std::vector<Geom> geoms;
+    std::vector<int> cell_off;   // [nrep + 1]
+    std::vector<int64_t> rep_n;  // molecules per replica (nrep > 1)
+    Geom *d_geoms = nullptr;
+    int *d_cell_off = nullptr, *d_cell_rep = nullptr;  // cell_rep: null for one replica
+    int *d_mol_rep = nullptr;                          // state; null for one replica
+    int *d_lf_chunk = nullptr, *d_lf_chunk_end = nullptr, *d_lf_off = nullptr;
+    int lf_nchunk = 0;
+    // pair-kernel launch shape: cells per work item, blocks per XCD
+    // (0: one block per work item)
+    int cpb = 1, blocks_per_xcd = 0;
     // state
     int n = 0;
     int nsites[kTypes] = {0, 0, 0, 0};
@@ -309,8 +323,8 @@ int upload_tables(mdx_context *c) {
 
 // LinkedCells ctor (cell_mode 0, LinkedCells.cpp:38-106) or rebuild
 // (cell_mode 1, float cell edge, LinkedCells.cpp:132-201).
-int build_geom(mdx_context *c, const double *box, double rc, int periodic, int cell_mode) {
-    Geom &g = c->g;
+int make_geom(mdx_context *c, const double *box, double rc, int periodic, int cell_mode, Geom &g,
+              long &ncell_out) {
     memset(&g, 0, sizeof(g));
     long ncell = 1;
     for (int d = 0; d < 3; ++d) {
@@ -335,15 +349,53 @@ int build_geom(mdx_context *c, const double *box, double rc, int periodic, int c
     }
     if (ncell > (1L << 28)) return fail(c, MDX_ERR_CAPACITY, "too many cells");
     g.periodic = periodic ? 1 : 0;
-    c->ncell = (int)ncell;
+    ncell_out = ncell;
     return MDX_OK;
 }
 
 void set_cutoff_terms(mdx_context *c) {
-    c->g.rc2 = c->rc * c->rc;
-    c->g.rclj2 = c->rc_lj * c->rc_lj;
-    // VectorizedCellProcessor ctor, VectorizedCellProcessor.cpp:21
-    c->g.epsRFInvrc3 = 2. * (c->eps_rf - 1.) / ((c->rc * c->rc * c->rc) * (2. * c->eps_rf + 1.));
+    for (Geom *g : {&c->g}) {
+        g->rc2 = c->rc * c->rc;
+        g->rclj2 = c->rc_lj * c->rc_lj;
+        // VectorizedCellProcessor ctor, VectorizedCellProcessor.cpp:21
+        g->epsRFInvrc3 =
+            2. * (c->eps_rf - 1.) / ((c->rc * c->rc * c->rc) * (2. * c->eps_rf + 1.));
+    }
+    for (Geom &g : c->geoms) {
+        g.rc2 = c->g.rc2;
+        g.rclj2 = c->g.rclj2;
+        g.epsRFInvrc3 = c->g.epsRFInvrc3;
+    }
+}
+
+// Replica geometries on the device; reduction buffers sized for nrep.
+int upload_reps(mdx_context *c) {
+    const int R = c->nrep;
+    TRY(dalloc(c, &c->d_geoms, R));
+    TRY(dalloc(c, &c->d_cell_off, R + 1));
+    TRY(dalloc(c, &c->red, 12 * (size_t)R));
+    HIPCHK(c, hipMemcpy(c->d_geoms, c->geoms.data(), sizeof(Geom) * R, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_cell_off, c->cell_off.data(), 4 * (size_t)(R + 1),
+                        hipMemcpyHostToDevice));
+    c->d_cell_rep = nullptr;
+    if (R > 1) {
+        std::vector<int> cr((size_t)c->ncell);
+        for (int r = 0; r < R; ++r)
+            for (int k = c->cell_off[r]; k < c->cell_off[r + 1]; ++k) cr[k] = r;
+        TRY(dalloc(c, &c->d_cell_rep, c->ncell));
+        HIPCHK(c, hipMemcpy(c->d_cell_rep, cr.data(), 4 * cr.size(), hipMemcpyHostToDevice));
+    }
+    return MDX_OK;
+}
+
+Reps reps(const mdx_context *c) {
+    Reps r;
+    r.geom = c->d_geoms;
+    r.cell_off = c->d_cell_off;
+    r.mol_rep = c->d_mol_rep;
+    r.cell_rep = c->d_cell_rep;
+    r.nrep = c->nrep;
+    return r;
 }
 
 int alloc_mol(mdx_context *c, MolArrays &m, int n) {
@@ -411,10 +463,14 @@ int scan_exclusive(mdx_context *c, const int *in, int *out, int n, int level = 0
     return MDX_OK;
 }
 
-// one 64-lane block per cell, launched as 8 contiguous ranges (one per XCD)
-int pair_grid(const mdx_context *c, int *per_xcd) {
-    *per_xcd = (c->ncell + 7) / 8;
-    return 8 * *per_xcd;
+// Work items of cpb consecutive cells in 8 contiguous ranges (one per XCD);
+// one block per item, or blocks_per_xcd persistent blocks per XCD.
+int pair_grid(const mdx_context *c, int *items_per_xcd) {
+    const int items = (c->ncell + c->cpb - 1) / c->cpb;
+    *items_per_xcd = (items + 7) / 8;
+    int per = *items_per_xcd;
+    if (c->blocks_per_xcd > 0) per = std::min(per, c->blocks_per_xcd);
+    return 8 * std::max(per, 1);
 }
 
 int n_present(const mdx_context *c) {
@@ -445,7 +501,7 @@ int launch_update(mdx_context *c) {
     MolArrays &src = c->mol[c->cur], &dst = c->mol[1 - c->cur];
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(c->ncell + 1)), dim3(kBlock), 0, c->stream, c->count,
                        c->ncell + 1, 0);
-    hipLaunchKernelGGL(k_bin_count, dim3(nblk(n)), dim3(kBlock), 0, c->stream, src, c->g, n,
+    hipLaunchKernelGGL(k_bin_count, dim3(nblk(n)), dim3(kBlock), 0, c->stream, src, reps(c), n,
                        c->cell, c->rank, c->count, c->ls1id);
     TRY(scan_exclusive(c, c->count, c->start, c->ncell));
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, c->cell,
@@ -462,7 +518,7 @@ int launch_update(mdx_context *c) {
                            c->ncell, c->start, c->sites[t].mstart, c->sites[t].cstart);
     }
     hipLaunchKernelGGL(k_build_sites, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, c->mol[c->cur],
-                       c->g, c->mcell, c->d_comps, c->sites[0], c->sites[1], c->sites[2],
+                       reps(c), c->mcell, c->d_comps, c->sites[0], c->sites[1], c->sites[2],
                        c->sites[3]);
     HIPCHK(c, hipGetLastError());
     return MDX_OK;
@@ -471,7 +527,10 @@ int launch_update(mdx_context *c) {
 int launch_traverse(mdx_context *c) {
     PairArgs a;
     memset(&a, 0, sizeof(a));
-    a.g = c->g;
+    a.R = reps(c);
+    a.rc2 = c->g.rc2;
+    a.rclj2 = c->g.rclj2;
+    a.epsRFInvrc3 = c->g.epsRFInvrc3;
     for (int t = 0; t < kTypes; ++t) {
         a.s[t] = c->sites[t];
         a.present[t] = c->nsites[t] > 0;
@@ -484,43 +543,58 @@ int launch_traverse(mdx_context *c) {
     a.pp.fshift6 = c->d_fshift6;
     a.pp.nlj = c->model.nlj;
     a.ncell = c->ncell;
-    int grid = pair_grid(c, &a.nblk_per_xcd);
+    a.cpb = c->cpb;
+    int grid = pair_grid(c, &a.items_per_xcd);
     a.counters = c->d_counters;
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    // one [ncell][4] partial array: the first i-site type writes each cell's
+    // row, later types add to it (the launches are ordered on the stream)
     int slot = 0;
     for (int t = 0; t < kTypes; ++t) {
         if (c->nsites[t] == 0) continue;
-        a.macro_part = c->macro_part + (size_t)slot * grid * 4;
+        a.macro_part = c->macro_part;
+        a.accumulate = slot > 0;
         PairKernel k = pair_kernel(t, c->precision == MDX_PREC_FP32_FP64ACC, c->counting,
                                    t == T_LJ && c->lj_at_origin, c->model.nlj > 1);
         hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, c->stream, a);
         ++slot;
     }
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    launch_reduce(c, c->macro_part, slot * grid, 4, c->red);
+    const int R = c->nrep;
+    if (slot == 0)
+        hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(64), 0, c->stream, c->red, 4 * R, 0.0);
+    else if (R == 1)
+        launch_reduce(c, c->macro_part, c->ncell, 4, c->red);
+    else
+        hipLaunchKernelGGL(k_seg_reduce, dim3(R), dim3(256), 0, c->stream, c->macro_part, 4,
+                           c->d_cell_off, c->red);
     hipLaunchKernelGGL(k_calc_fm, dim3(nblk(c->n)), dim3(kBlock), 0, c->stream, c->n,
                        c->mol[c->cur], c->d_comps, c->sites[0], c->sites[1], c->sites[2],
                        c->sites[3], c->F, c->M, c->Vi);
-    hipLaunchKernelGGL(k_finalize_macro, dim3(1), dim3(1), 0, c->stream, c->red, c->red + 4,
-                       c->step_out, c->step_idx);
+    hipLaunchKernelGGL(k_finalize_macro, dim3((R + 63) / 64), dim3(64), 0, c->stream, c->red,
+                       c->red + 4 * R, c->step_out, c->step_idx, R);
     HIPCHK(c, hipGetLastError());
     return MDX_OK;
 }
 
 int launch_lf_pre(mdx_context *c, double dt) {
     hipLaunchKernelGGL(k_leapfrog_pre, dim3(nblk(c->n)), dim3(kBlock), 0, c->stream,
-                       c->mol[c->cur], c->F, c->M, c->d_comps, c->g, dt, c->n);
+                       c->mol[c->cur], c->F, c->M, c->d_comps, reps(c), dt, c->n);
     HIPCHK(c, hipGetLastError());
     return MDX_OK;
 }
 
 int launch_lf_post(mdx_context *c, double dt) {
-    int nb = nblk(c->n);
+    const int R = c->nrep, nb = c->lf_nchunk;
     hipLaunchKernelGGL(k_leapfrog_post, dim3(nb), dim3(kBlock), 0, c->stream, c->mol[c->cur], c->F,
-                       c->M, c->d_comps, dt, c->n, c->lf_part);
-    launch_reduce(c, c->lf_part, nb, 4, c->red + 8);
-    hipLaunchKernelGGL(k_finalize_lf, dim3(1), dim3(1), 0, c->stream, c->red + 8, c->step_out,
-                       c->step_idx);
+                       c->M, c->d_comps, dt, c->n, c->d_lf_chunk, c->d_lf_chunk_end, c->lf_part);
+    if (R == 1)
+        launch_reduce(c, c->lf_part, nb, 4, c->red + 8);
+    else
+        hipLaunchKernelGGL(k_seg_reduce, dim3(R), dim3(256), 0, c->stream, c->lf_part, 4,
+                           c->d_lf_off, c->red + 8 * R);
+    hipLaunchKernelGGL(k_finalize_lf, dim3((R + 63) / 64), dim3(64), 0, c->stream, c->red + 8 * R,
+                       c->step_out, c->step_idx, R);
     HIPCHK(c, hipGetLastError());
     return MDX_OK;
 }
@@ -546,7 +620,7 @@ int ensure_steps(mdx_context *c, int nsteps) {
     if (nsteps + 1 <= c->cap_steps) return MDX_OK;
     int cap = std::max(nsteps + 1, 2 * c->cap_steps);
     double *p;
-    TRY(dalloc(c, &p, (size_t)6 * cap));
+    TRY(dalloc(c, &p, (size_t)6 * cap * c->nrep));
     c->step_out = p;  // the old buffer stays owned by the context until destroy
     c->cap_steps = cap;
     drop_graphs(c);
@@ -555,6 +629,11 @@ int ensure_steps(mdx_context *c, int nsteps) {
 
 int state_setup(mdx_context *c, int64_t n64, const int32_t *cid_host) {
     if (n64 <= 0 || n64 > (1 << 27)) return fail(c, MDX_ERR_ARG, "molecule count out of range");
+    if (c->nrep > 1) {
+        int64_t tot = 0;
+        for (int64_t k : c->rep_n) tot += k;
+        if (tot != n64) return fail(c, MDX_ERR_ARG, "n differs from the sum of replica counts");
+    }
     const int n = (int)n64;
     drop_graphs(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -587,10 +666,44 @@ int state_setup(mdx_context *c, int64_t n64, const int32_t *cid_host) {
         TRY(alloc_sites(c, c->sites[t], c->nsites[t], c->ncell, n, t));
     }
     TRY(alloc_scan(c, std::max(n, c->ncell) + 1));
-    int px;
-    int grid = pair_grid(c, &px);
-    TRY(dalloc(c, &c->macro_part, (size_t)grid * 4 * kTypes, true));
-    TRY(dalloc(c, &c->lf_part, (size_t)nblk(n) * 4, true));
+    TRY(dalloc(c, &c->macro_part, (size_t)c->ncell * 4, true));
+    // molecule chunks of <= kBlock that never straddle two replicas
+    {
+        std::vector<int> ch, che, off(1, 0);
+        std::vector<int64_t> rn = c->nrep > 1 ? c->rep_n : std::vector<int64_t>(1, n);
+        int64_t m0 = 0;
+        for (int64_t cnt : rn) {
+            for (int64_t k = 0; k < cnt; k += kBlock) {
+                ch.push_back((int)(m0 + k));
+                che.push_back((int)(m0 + cnt));
+            }
+            m0 += cnt;
+            off.push_back((int)ch.size());
+        }
+        if (ch.empty()) {
+            ch.push_back(0);
+            che.push_back(0);
+        }
+        c->lf_nchunk = (int)ch.size();
+        TRY(dalloc(c, &c->d_lf_chunk, ch.size(), true));
+        TRY(dalloc(c, &c->d_lf_chunk_end, ch.size(), true));
+        TRY(dalloc(c, &c->d_lf_off, off.size(), true));
+        HIPCHK(c, hipMemcpy(c->d_lf_chunk, ch.data(), 4 * ch.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_lf_chunk_end, che.data(), 4 * che.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_lf_off, off.data(), 4 * off.size(), hipMemcpyHostToDevice));
+        TRY(dalloc(c, &c->lf_part, (size_t)c->lf_nchunk * 4, true));
+    }
+    c->d_mol_rep = nullptr;
+    if (c->nrep > 1) {
+        std::vector<int> mr((size_t)n);
+        int64_t m0 = 0;
+        for (int r = 0; r < c->nrep; ++r) {
+            for (int64_t k = 0; k < c->rep_n[r]; ++k) mr[m0 + k] = r;
+            m0 += c->rep_n[r];
+        }
+        TRY(dalloc(c, &c->d_mol_rep, n, true));
+        HIPCHK(c, hipMemcpy(c->d_mol_rep, mr.data(), 4 * mr.size(), hipMemcpyHostToDevice));
+    }
     TRY(dalloc(c, &c->dl_tmp, (size_t)n * 4, true));
     c->adj_site = c->adj_cot = c->adj_part = c->adj_out = nullptr;  // made on first use
     HIPCHK(c, hipMemsetAsync(c->F, 0, 3 * (size_t)n * 8, c->stream));
@@ -628,7 +741,7 @@ int mdx_create(int device, mdx_context **out) {
         return MDX_ERR_HIP;
     }
     for (auto &e : c->ev) (void)hipEventCreate(&e);
-    if (dalloc(c, &c->red, 64) || dalloc(c, &c->red_tmp, 256 * 8) || dalloc(c, &c->d_counters, 8) || dalloc(c, &c->step_idx, 1) ||
+    if (dalloc(c, &c->red_tmp, 256 * 8) || dalloc(c, &c->d_counters, 8) || dalloc(c, &c->step_idx, 1) ||
         dalloc(c, &c->d_comps, kMaxComp) || ensure_steps(c, 1)) {
         g_create_error = c->err;
         free_list(c->allocs);
@@ -690,11 +803,112 @@ int mdx_set_domain(mdx_context *c, const double *box, double rc, double rc_lj, i
         return fail(c, MDX_ERR_ARG, "rcLJ larger than the cell cutoff rc is not supported");
     c->rc = rc;
     c->rc_lj = rc_lj;
-    TRY(build_geom(c, box, rc, periodic, cell_mode));
+    long nc = 0;
+    TRY(make_geom(c, box, rc, periodic, cell_mode, c->g, nc));
+    c->ncell = (int)nc;
+    c->nrep = 1;
+    c->geoms.assign(1, c->g);
+    c->cell_off = {0, c->ncell};
+    c->rep_n.clear();
+    drop_graphs(c);
     set_cutoff_terms(c);
+    TRY(upload_reps(c));
     build_tables(c);
     TRY(upload_tables(c));
     c->have_domain = true;
+    return MDX_OK;
+}
+
+int mdx_set_replicas(mdx_context *c, int nrep, const double *boxes, const int64_t *counts,
+                     double rc, double rc_lj, int periodic, int cell_mode) {
+    if (!c || !boxes || !counts || nrep < 1) return MDX_ERR_ARG;
+    if (!c->have_model) return fail(c, MDX_ERR_STATE, "set components before the replicas");
+    if (c->have_state) return fail(c, MDX_ERR_STATE, "replicas must be set before upload");
+    if (!(rc > 0) || !(rc_lj > 0)) return fail(c, MDX_ERR_ARG, "cutoffs must be positive");
+    if (rc_lj > rc)
+        return fail(c, MDX_ERR_ARG, "rcLJ larger than the cell cutoff rc is not supported");
+    c->rc = rc;
+    c->rc_lj = rc_lj;
+    std::vector<Geom> gs((size_t)nrep);
+    std::vector<int> off(1, 0);
+    long tot = 0;
+    for (int r = 0; r < nrep; ++r) {
+        if (counts[r] < 0) return fail(c, MDX_ERR_ARG, "negative replica count");
+        long nc = 0;
+        TRY(make_geom(c, boxes + 3 * r, rc, periodic, cell_mode, gs[r], nc));
+        tot += nc;
+        if (tot > (1L << 28)) return fail(c, MDX_ERR_CAPACITY, "too many cells");
+        off.push_back((int)tot);
+    }
+    c->geoms = gs;
+    c->g = gs[0];
+    c->cell_off = off;
+    c->ncell = (int)tot;
+    c->nrep = nrep;
+    c->rep_n.assign(counts, counts + nrep);
+    drop_graphs(c);
+    set_cutoff_terms(c);
+    build_tables(c);
+    TRY(upload_tables(c));
+    TRY(upload_reps(c));
+    c->have_domain = true;
+    return MDX_OK;
+}
+
+int mdx_set_pair_config(mdx_context *c, int cells_per_block, int blocks_per_xcd) {
+    if (!c || cells_per_block < 1 || cells_per_block > 64 || blocks_per_xcd < 0)
+        return fail(c, MDX_ERR_ARG, "cells_per_block in [1, 64], blocks_per_xcd >= 0");
+    c->cpb = cells_per_block;
+    c->blocks_per_xcd = blocks_per_xcd;
+    drop_graphs(c);
+    return MDX_OK;
+}
+
+int mdx_autotune(mdx_context *c, int reps, int32_t *chosen, double *best_ms) {
+    if (!c) return MDX_ERR_ARG;
+    TRY(need(c, true, true, true));
+    HIPCHK(c, hipSetDevice(c->device));
+    if (reps < 1) reps = 3;
+    const bool was_timing = c->timing, was_counting = c->counting;
+    c->timing = true;
+    c->counting = false;
+    const int cpbs[] = {1, 2, 4, 8};
+    const int bpx[] = {0, 64, 128, 256, 512};
+    double best = 1e300;
+    int bc = c->cpb, bb = c->blocks_per_xcd;
+    for (int cp : cpbs)
+        for (int b : bpx) {
+            c->cpb = cp;
+            c->blocks_per_xcd = b;
+            int ipx = 0;
+            (void)pair_grid(c, &ipx);
+            if (b > 0 && b >= ipx) continue;  // same as one block per item
+            double t = 0;
+            for (int k = 0; k <= reps; ++k) {
+                HIPCHK(c, hipMemsetAsync(c->step_idx, 0, sizeof(int), c->stream));
+                TRY(launch_traverse(c));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                float ms = 0;
+                HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+                if (k > 0) t += ms;  // first run warms up
+            }
+            t /= reps;
+            if (t < best) {
+                best = t;
+                bc = cp;
+                bb = b;
+            }
+        }
+    c->cpb = bc;
+    c->blocks_per_xcd = bb;
+    c->timing = was_timing;
+    c->counting = was_counting;
+    drop_graphs(c);
+    if (chosen) {
+        chosen[0] = bc;
+        chosen[1] = bb;
+    }
+    if (best_ms) *best_ms = best;
     return MDX_OK;
 }
 
@@ -863,8 +1077,8 @@ int mdx_traverse(mdx_context *c, double *macro) {
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     if (macro) {
         TRY(0);
-        HIPCHK(c, hipMemcpyAsync(macro, c->red + 4, 4 * sizeof(double), hipMemcpyDeviceToHost,
-                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(macro, c->red + 4 * c->nrep, 4 * sizeof(double) * c->nrep,
+                                 hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->timing) {
@@ -891,8 +1105,8 @@ int mdx_leapfrog_post(mdx_context *c, double dt, double *sums) {
     HIPCHK(c, hipMemsetAsync(c->step_idx, 0, sizeof(int), c->stream));
     TRY(launch_lf_post(c, dt));
     if (sums) {
-        HIPCHK(c, hipMemcpyAsync(sums, c->red + 8, 4 * sizeof(double), hipMemcpyDeviceToHost,
-                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(sums, c->red + 8 * c->nrep, 4 * sizeof(double) * c->nrep,
+                                 hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MDX_OK;
@@ -930,7 +1144,8 @@ int mdx_step(mdx_context *c, int nsteps, double dt, double *per_step, int use_gr
     }
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     if (per_step && nsteps > 0)
-        HIPCHK(c, hipMemcpyAsync(per_step, c->step_out, (size_t)nsteps * 6 * sizeof(double),
+        HIPCHK(c, hipMemcpyAsync(per_step, c->step_out,
+                                 (size_t)nsteps * 6 * c->nrep * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->timing) {
@@ -950,6 +1165,7 @@ int mdx_synchronize(mdx_context *c) {
 int mdx_cell_ids(mdx_context *c, int64_t n, int64_t *out) {
     if (!c || !out) return MDX_ERR_ARG;
     TRY(need(c, true, true, true));
+    if (c->nrep > 1) return fail(c, MDX_ERR_ARG, "not available for batched replicas");
     if (n != c->n) return fail(c, MDX_ERR_ARG, "n differs from the uploaded count");
     std::vector<int> tmp((size_t)n), orig((size_t)n);
     HIPCHK(c, hipMemcpyAsync(tmp.data(), c->mls1id, (size_t)n * 4, hipMemcpyDeviceToHost,
@@ -964,6 +1180,7 @@ int mdx_cell_ids(mdx_context *c, int64_t n, int64_t *out) {
 int mdx_cell_geometry(mdx_context *c, int32_t *cells_per_dim, double *cell_length) {
     if (!c) return MDX_ERR_ARG;
     TRY(need(c, true, true, false));
+    if (c->nrep > 1) return fail(c, MDX_ERR_ARG, "not available for batched replicas");
     for (int d = 0; d < 3; ++d) {
         if (cells_per_dim) cells_per_dim[d] = c->g.cpd[d];
         if (cell_length) cell_length[d] = c->g.cl[d];
@@ -974,6 +1191,7 @@ int mdx_cell_geometry(mdx_context *c, int32_t *cells_per_dim, double *cell_lengt
 int mdx_cell_pairs(mdx_context *c, int32_t *pairs, int64_t max_pairs, int64_t *count) {
     if (!c || !count) return MDX_ERR_ARG;
     TRY(need(c, true, true, false));
+    if (c->nrep > 1) return fail(c, MDX_ERR_ARG, "not available for batched replicas");
     HIPCHK(c, hipSetDevice(c->device));
     long tot = (long)c->ncell * 27;
     int *d;
@@ -1052,6 +1270,7 @@ int mdx_adjoint(mdx_context *c, int64_t n, const double *cot, double *dU_deps, d
     TRY(need(c, true, true, true));
     if (n != c->n) return fail(c, MDX_ERR_ARG, "n differs from the uploaded count");
     if (vjp && !cot) return fail(c, MDX_ERR_ARG, "vjp requested without a cotangent");
+    if (c->nrep > 1) return fail(c, MDX_ERR_ARG, "not available for batched replicas");
     if (c->nsites[T_D] > 0 || c->nsites[T_Q] > 0)
         return fail(c, MDX_ERR_ARG, "the adjoint covers LJ and charge sites only");
     const int nl = c->model.nlj;

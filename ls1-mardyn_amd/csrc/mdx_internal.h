@@ -81,14 +81,37 @@ struct PairParams {
 };
 
 
+// Replica table: independent systems (same components and cutoffs, own box
+// and cells) stored back to back. Replica r owns the global cells
+// [cell_off[r], cell_off[r+1]) and, because the counting sort orders
+// molecules by global cell, always the same molecule index range. A single
+// system is the one-replica case (mol_rep / cell_rep null).
+struct Reps {
+    const Geom *geom;      // [nrep]
+    const int *cell_off;   // [nrep + 1]
+    const int *mol_rep;    // [n] replica of each molecule, or null
+    const int *cell_rep;   // [ncell] replica of each global cell, or null
+    int nrep;
+};
+
+__device__ __forceinline__ int mol_replica(const Reps &R, int i) {
+    return R.mol_rep ? R.mol_rep[i] : 0;
+}
+__device__ __forceinline__ int cell_replica(const Reps &R, int c) {
+    return R.cell_rep ? R.cell_rep[c] : 0;
+}
+
 struct PairArgs {
-    Geom g;
+    Reps R;
+    double rc2, rclj2, epsRFInvrc3;   // shared by all replicas
     SiteArrays s[kTypes];
     PairParams pp;
-    int ncell;
-    int nblk_per_xcd;   // cells per XCD (one 64-lane block per cell; launch = 8 * nblk_per_xcd)
+    int ncell;          // global cells
+    int cpb;            // cells per work item (consecutive cells, one block)
+    int items_per_xcd;  // work items of each XCD's contiguous range
     int present[kTypes];
-    double *macro_part; // [launch blocks][4]
+    double *macro_part; // [ncell][4] per-cell sums {u6 (LJ), uXpoles, virial, rf}
+    int accumulate;     // add to macro_part rows (a later i-site type) instead of writing
     unsigned long long *counters;  // [8] or null
 };
 

@@ -112,10 +112,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 // DomainDecompBase::handleDomainLeavingParticles (DomainDecompBase.cpp:199-249).
 __global__ __launch_bounds__(256) void k_leapfrog_pre(MolArrays s, const double *__restrict__ F,
                                                       const double *__restrict__ M,
-                                                      const Component *__restrict__ comps, Geom g,
+                                                      const Component *__restrict__ comps, Reps R,
                                                       double dt, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const Geom &g = R.geom[mol_replica(R, i)];
     const Component &c = comps[s.cid[i]];
     double dt_halve = .5 * dt;
     double dtInv2m = dt_halve / c.m;
@@ -162,14 +163,19 @@ __global__ __launch_bounds__(256) void k_leapfrog_pre(MolArrays s, const double 
 
 // FullMolecule::upd_postF (FullMolecule.cpp:363-387) with per-block FP64
 // partial sums {sum m v^2, sum I w^2, N, rotDOF} (Leapfrog.cpp:63-147).
+// Block b covers molecules [chunk[b], chunk[b] + 256) clipped to
+// chunk_end[b]: chunks never straddle two replicas.
 __global__ __launch_bounds__(256) void k_leapfrog_post(MolArrays s, const double *__restrict__ F,
                                                        const double *__restrict__ M,
                                                        const Component *__restrict__ comps,
-                                                       double dt, int n, double *partial) {
+                                                       double dt, int n,
+                                                       const int *__restrict__ chunk,
+                                                       const int *__restrict__ chunk_end,
+                                                       double *partial) {
     __shared__ double red[4][4];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = chunk[blockIdx.x] + threadIdx.x;
     double mv2 = 0, iw2 = 0, cnt = 0, rot = 0;
-    if (i < n) {
+    if (i < chunk_end[blockIdx.x]) {
         const Component &c = comps[s.cid[i]];
         double dt_halve = 0.5 * dt;
         double dtInv2m = dt_halve / c.m;
@@ -241,20 +247,27 @@ __global__ void k_fill_int(int *a, int n, int v) {
     if (i < n) a[i] = v;
 }
 
+__global__ void k_fill_double(double *a, int n, double v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 // Cell of every molecule (ls1 index -> inner index) and its arrival rank in
 // the cell. Lanes of a wavefront that share a cell (molecules are stored in
 // cell order, so a wave touches only a few cells) take one atomic per cell.
-__global__ __launch_bounds__(256) void k_bin_count(MolArrays s, Geom g, int n, int *cell,
+__global__ __launch_bounds__(256) void k_bin_count(MolArrays s, Reps R, int n, int *cell,
                                                    int *rank, int *count, int *ls1id) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool in = i < n;
     int c = -1;
     if (in) {
+        const int rep = mol_replica(R, i);
+        const Geom &g = R.geom[rep];
         double p[3] = {s.rx[i], s.ry[i], s.rz[i]};
         int ci[3];
         ls1_cell3(g, p, ci);
-        c = inner_index(g, ci);
+        c = R.cell_off[rep] + inner_index(g, ci);
         cell[i] = c;
         ls1id[i] = (ci[2] * g.cpd[1] + ci[1]) * g.cpd[0] + ci[0];
     }
@@ -373,7 +386,7 @@ __global__ void k_cell_site_start(int ncell, const int *__restrict__ start,
 // reference mutates the state here, Quaternion.h:40-42), rotate the body-frame
 // sites, record molecule position and parameters; positions also relative to
 // the molecule's cell corner in FP32.
-__global__ __launch_bounds__(256) void k_build_sites(int n, MolArrays s, Geom g,
+__global__ __launch_bounds__(256) void k_build_sites(int n, MolArrays s, Reps R,
                                                      const int *__restrict__ mcell,
                                                      const Component *__restrict__ comps,
                                                      SiteArrays s0, SiteArrays s1, SiteArrays s2,
@@ -386,7 +399,9 @@ __global__ __launch_bounds__(256) void k_build_sites(int n, MolArrays s, Geom g,
     s.qw[i] = q[0]; s.qx[i] = q[1]; s.qy[i] = q[2]; s.qz[i] = q[3];
     const Component &c = comps[s.cid[i]];
     double r[3] = {s.rx[i], s.ry[i], s.rz[i]};
-    int cell = mcell[i];
+    const int rep = mol_replica(R, i);
+    const Geom &g = R.geom[rep];
+    int cell = mcell[i] - R.cell_off[rep];
     int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
     double corner[3] = {g.bbmin[0] + cx * g.cl[0], g.bbmin[1] + cy * g.cl[1],
                         g.bbmin[2] + cz * g.cl[2]};
@@ -468,26 +483,53 @@ __global__ __launch_bounds__(256) void k_calc_fm(int n, MolArrays s,
 // sides of every pair {u6, uXpoles, virial, rf}; halved here. myRF carries
 // the reference's negative sign (VectorizedCellProcessor.cpp:2727).
 // Step record: Upot, virial, summv2, sumIw2, Upot_LJ, Upot_rest.
+// One thread per replica; raw, macro are [nrep][4], the step record is
+// [step][nrep][6].
 __global__ void k_finalize_macro(const double *raw, double *macro, double *step_out,
-                                 const int *step_idx) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+                                 const int *step_idx, int nrep) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrep) return;
+    raw += 4 * r;
+    macro += 4 * r;
     double u6 = 0.5 * raw[0], ux = 0.5 * raw[1], vir = 0.5 * raw[2], rf = -0.5 * raw[3];
     macro[0] = u6;
     macro[1] = ux;
     macro[2] = vir;
     macro[3] = rf;
-    double *rec = step_out + 6 * step_idx[0];
+    double *rec = step_out + 6 * ((size_t)step_idx[0] * nrep + r);
     rec[0] = u6 / 6.0 + ux + rf;
     rec[1] = vir + 3.0 * rf;
     rec[4] = u6 / 6.0;
     rec[5] = ux + rf;
 }
 
-__global__ void k_finalize_lf(const double *sums, double *step_out, const int *step_idx) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double *rec = step_out + 6 * step_idx[0];
-    rec[2] = sums[0];
-    rec[3] = sums[1];
+__global__ void k_finalize_lf(const double *sums, double *step_out, const int *step_idx,
+                              int nrep) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrep) return;
+    double *rec = step_out + 6 * ((size_t)step_idx[0] * nrep + r);
+    rec[2] = sums[4 * r];
+    rec[3] = sums[4 * r + 1];
+}
+
+// Segmented fixed-order sum: block r adds rows [off[r], off[r+1]) of a
+// [rows][width] array into out[r][width] (one block per replica).
+__global__ __launch_bounds__(256) void k_seg_reduce(const double *__restrict__ in, int width,
+                                                    const int *__restrict__ off, double *out) {
+    __shared__ double red[256];
+    const int b0 = off[blockIdx.x], b1 = off[blockIdx.x + 1];
+    for (int w = 0; w < width; ++w) {
+        double a = 0;
+        for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) a += in[(size_t)b * width + w];
+        red[threadIdx.x] = a;
+        __syncthreads();
+        for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[(size_t)blockIdx.x * width + w] = red[0];
+        __syncthreads();
+    }
 }
 
 __global__ void k_step_advance(int *step_idx) {

@@ -69,7 +69,7 @@ __device__ __forceinline__ void eval_pair(const PairArgs &a, const ISite<R> &I,
             sign = R(-1);
             tq = o.M2;
         } else if constexpr (TI == T_D && TJ == T_D) {
-            body_dd<R>(d, I.e, I.par, ej, qj, (R)a.g.epsRFInvrc3, o);
+            body_dd<R>(d, I.e, I.par, ej, qj, (R)a.epsRFInvrc3, o);
             tq = o.M1;
         } else if constexpr (TI == T_D && TJ == T_Q) {
             body_dq<R>(d, I.e, I.par, ej, qj, o);
@@ -127,22 +127,23 @@ constexpr int kUnroll = 4;  // stage-A candidates in flight per lane
 // from LDS (kUnroll candidates per lane in flight) and keeps per-lane hit
 // lists, (4) stage B evaluates the bodies for the hits.
 template <int TI, int TJ, typename R, bool MOLSITE, bool MULTI, bool COUNT>
-__device__ __forceinline__ void partner_pass(const PairArgs &a, const ISite<R> &I, bool valid,
+__device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, int cbase,
+                                             const ISite<R> &I, bool valid,
                                              int cx, int cy, int cz, int ph, int P, int lane,
                                              Tile<TI, R, MOLSITE, MULTI> &T, Acc &acc, R eps24,
                                              R sig2, R shift6, unsigned long long &ntest) {
     using C = TileCfg<TI, R, MOLSITE, MULTI>;
     const SiteArrays &J = a.s[TJ];
-    const double rc2 = (TI == T_LJ) ? a.g.rclj2 : a.g.rc2;
+    const double rc2 = (TI == T_LJ) ? a.rclj2 : a.rc2;
     const R rc2r = (R)rc2;
     unsigned long long nhit = 0;
     int nk = 0;
     if (lane < 27) {
-        Nbr b = neighbour(a.g, cx, cy, cz, lane);
+        Nbr b = neighbour(g, cx, cy, cz, lane);
         int j0 = 0;
         if (b.ok) {
-            j0 = J.cstart[b.cell];
-            nk = J.cstart[b.cell + 1] - j0;
+            j0 = J.cstart[cbase + b.cell];
+            nk = J.cstart[cbase + b.cell + 1] - j0;
         }
         T.j0[lane] = j0;
         T.shift[lane][0] = b.s[0];
@@ -165,7 +166,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const ISite<R> &
             const int idx = base + s;
             const int k = tile_k(T.pre, idx);
             const int j = T.j0[k] + (idx - T.pre[k]);
-            const V3<R> off = tile_offset<R>(a.g, T.shift, k);
+            const V3<R> off = tile_offset<R>(g, T.shift, k);
             const V3<R> p = SiteView<R>::pos(J, j);
             Q4<R> q;
             q.x = p.x + off.x;
@@ -256,22 +257,20 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const ISite<R> &
 
 __device__ __forceinline__ double shfl_xor_d(double v, int o) { return __shfl_xor(v, o, 64); }
 
-// One 64-lane block (one wavefront) per cell.
+// All i sites of one cell (global index) with their partners; writes the
+// cell's macroscopic sums to macro_part[4 * cell].
 template <int TI, typename R, bool COUNT, bool MOLSITE, bool MULTI>
-__global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
-    __shared__ Tile<TI, R, MOLSITE, MULTI> T;
-    const int lane = threadIdx.x;
-    // XCD-aware order: hardware block b runs on XCD b % 8; each XCD gets a
-    // contiguous range of cells so neighbour data is shared in its L2.
-    const int hb = blockIdx.x;
-    const int cell = (hb & 7) * a.nblk_per_xcd + (hb >> 3);
+__device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE, MULTI> &T,
+                                          int cell, int lane, unsigned long long &ntest) {
     double wmac[3] = {0, 0, 0};
-    unsigned long long ntest = 0;
-    if (cell < a.ncell) {
+    {
+        const int rep = cell_replica(a.R, cell);
+        const Geom &g = a.R.geom[rep];
+        const int cbase = a.R.cell_off[rep], local = cell - cbase;
         const SiteArrays &S = a.s[TI];
         const int i_begin = S.cstart[cell], i_end = S.cstart[cell + 1];
-        const int cx = cell % a.g.n[0], cy = (cell / a.g.n[0]) % a.g.n[1],
-                  cz = cell / (a.g.n[0] * a.g.n[1]);
+        const int cx = local % g.n[0], cy = (local / g.n[0]) % g.n[1],
+                  cz = local / (g.n[0] * g.n[1]);
         R eps24 = R(0), sig2 = R(0), shift6 = R(0);
         if constexpr (TI == T_LJ) {
             if constexpr (sizeof(R) == 8) {
@@ -303,20 +302,20 @@ __global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
             for (int k = 0; k < 3; ++k) acc.f[k] = acc.v[k] = acc.m[k] = 0.0;
             acc.u = acc.vir = acc.rf = 0.0;
             if constexpr (TI == T_LJ) {
-                partner_pass<T_LJ, T_LJ, R, MOLSITE, MULTI, COUNT>(a, I, valid, cx, cy, cz, ph, P,
+                partner_pass<T_LJ, T_LJ, R, MOLSITE, MULTI, COUNT>(a, g, cbase, I, valid, cx, cy, cz, ph, P,
                                                                    lane, T, acc, eps24, sig2,
                                                                    shift6, ntest);
             } else {
                 if (a.present[T_C])
-                    partner_pass<TI, T_C, R, MOLSITE, MULTI, COUNT>(a, I, valid, cx, cy, cz, ph, P,
+                    partner_pass<TI, T_C, R, MOLSITE, MULTI, COUNT>(a, g, cbase, I, valid, cx, cy, cz, ph, P,
                                                                     lane, T, acc, eps24, sig2,
                                                                     shift6, ntest);
                 if (a.present[T_D])
-                    partner_pass<TI, T_D, R, MOLSITE, MULTI, COUNT>(a, I, valid, cx, cy, cz, ph, P,
+                    partner_pass<TI, T_D, R, MOLSITE, MULTI, COUNT>(a, g, cbase, I, valid, cx, cy, cz, ph, P,
                                                                     lane, T, acc, eps24, sig2,
                                                                     shift6, ntest);
                 if (a.present[T_Q])
-                    partner_pass<TI, T_Q, R, MOLSITE, MULTI, COUNT>(a, I, valid, cx, cy, cz, ph, P,
+                    partner_pass<TI, T_Q, R, MOLSITE, MULTI, COUNT>(a, g, cbase, I, valid, cx, cy, cz, ph, P,
                                                                     lane, T, acc, eps24, sig2,
                                                                     shift6, ntest);
             }
@@ -341,16 +340,35 @@ __global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
     }
     for (int k = 0; k < 3; ++k)
         for (int o = 32; o > 0; o >>= 1) wmac[k] += shfl_xor_d(wmac[k], o);
+    if (lane == 0) {
+        double *o = a.macro_part + 4 * (size_t)cell;
+        const double v[4] = {(TI == T_LJ) ? wmac[0] : 0.0, (TI == T_LJ) ? 0.0 : wmac[0], wmac[1],
+                             wmac[2]};
+        for (int k = 0; k < 4; ++k) o[k] = a.accumulate ? o[k] + v[k] : v[k];
+    }
+}
+
+// One 64-lane block (one wavefront) per work item of cpb consecutive cells.
+// XCD-aware: hardware block b runs on XCD b % 8, and XCD x owns the
+// contiguous item range [x, x + 1) * items_per_xcd, so neighbouring cells
+// share its L2. A grid smaller than 8 * items_per_xcd makes the blocks
+// persistent: each walks its XCD's items with stride gridDim.x / 8, and every
+// block leaves after its last item.
+template <int TI, typename R, bool COUNT, bool MOLSITE, bool MULTI>
+__global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
+    __shared__ Tile<TI, R, MOLSITE, MULTI> T;
+    const int lane = threadIdx.x;
+    const int xcd = blockIdx.x & 7, stride = gridDim.x >> 3;
+    unsigned long long ntest = 0;
+    for (int item = blockIdx.x >> 3; item < a.items_per_xcd; item += stride) {
+        const int c0 = (xcd * a.items_per_xcd + item) * a.cpb;
+        const int c1 = min(c0 + a.cpb, a.ncell);
+        for (int cell = c0; cell < c1; ++cell)
+            pair_cell<TI, R, COUNT, MOLSITE, MULTI>(a, T, cell, lane, ntest);
+    }
     if (COUNT) {
         for (int o = 32; o > 0; o >>= 1) ntest += __shfl_xor(ntest, o, 64);
         if (lane == 0) atomicAdd(&a.counters[0], ntest);
-    }
-    if (lane == 0) {
-        double *o = a.macro_part + 4 * hb;
-        o[0] = (TI == T_LJ) ? wmac[0] : 0.0;
-        o[1] = (TI == T_LJ) ? 0.0 : wmac[0];
-        o[2] = wmac[1];
-        o[3] = wmac[2];
     }
 }
 

@@ -29,7 +29,7 @@ EXPORTS = (
     "mdx_traverse", "mdx_leapfrog_pre", "mdx_leapfrog_post", "mdx_step", "mdx_synchronize",
     "mdx_cell_ids", "mdx_cell_geometry", "mdx_cell_pairs", "mdx_set_counting", "mdx_counters",
     "mdx_upload_device", "mdx_set_timing", "mdx_last_kernel_ms", "mdx_set_charges",
-    "mdx_adjoint", "mdx_lj_tables",
+    "mdx_adjoint", "mdx_lj_tables", "mdx_set_replicas", "mdx_set_pair_config", "mdx_autotune",
 )
 
 _lib = None
@@ -73,6 +73,10 @@ def load():
     L.mdx_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.mdx_set_charges.argtypes = [vp, C.c_int, _dp]
     L.mdx_lj_tables.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+    L.mdx_set_replicas.argtypes = [vp, C.c_int, _dp, _lp, C.c_double, C.c_double, C.c_int,
+                                   C.c_int]
+    L.mdx_set_pair_config.argtypes = [vp, C.c_int, C.c_int]
+    L.mdx_autotune.argtypes = [vp, C.c_int, _ip, C.POINTER(C.c_double)]
     L.mdx_adjoint.argtypes = [vp, C.c_int64] + [vp] * 8
     _lib = L
     return L
@@ -94,6 +98,7 @@ class Context:
             raise MdxError(f"mdx_create failed ({rc}): {self.L.mdx_last_error(None).decode()}")
         self.h = h
         self.n = 0
+        self.nrep = 1
 
     def _chk(self, rc, what):
         if rc != 0:
@@ -130,6 +135,7 @@ class Context:
         self._chk(self.L.mdx_set_domain(self.h, np.ascontiguousarray(box, np.float64), float(rc),
                                         float(rc_lj), int(bool(periodic)), int(cell_mode)),
                   "mdx_set_domain")
+        self.nrep = 1
         self.set_precision(precision)
         return self
 
@@ -137,6 +143,37 @@ class Context:
         self.setup(s.ncomp, s.comp_desc, s.mix, s.eps_rf, s.box, s.rc, s.rc_lj, periodic,
                    cell_mode, precision)
         self.upload(s.r, s.v, s.q, s.L, s.cid)
+        return self
+
+    def set_replicas(self, boxes, counts, rc, rc_lj, periodic=True, cell_mode=0):
+        boxes = np.ascontiguousarray(boxes, np.float64).reshape(-1, 3)
+        counts = np.ascontiguousarray(counts, np.int64)
+        self._chk(self.L.mdx_set_replicas(self.h, len(boxes), boxes.ravel(), counts, float(rc),
+                                          float(rc_lj), int(bool(periodic)), int(cell_mode)),
+                  "mdx_set_replicas")
+        self.nrep = len(boxes)
+
+    def set_pair_config(self, cells_per_block=1, blocks_per_xcd=0):
+        self._chk(self.L.mdx_set_pair_config(self.h, int(cells_per_block), int(blocks_per_xcd)),
+                  "mdx_set_pair_config")
+
+    def autotune(self, reps=3):
+        ch = np.zeros(2, np.int32)
+        ms = C.c_double(0)
+        self._chk(self.L.mdx_autotune(self.h, int(reps), ch, C.byref(ms)), "mdx_autotune")
+        return (int(ch[0]), int(ch[1])), ms.value
+
+    def setup_replicas(self, systems, periodic=True, precision=PREC_FP64, cell_mode=0):
+        """Batched replicas from a list of systems that share components and
+        cutoffs (each keeps its own box and molecules)."""
+        s0 = systems[0]
+        desc = np.ascontiguousarray(s0.comp_desc, np.float64)
+        self.setup(s0.ncomp, desc, s0.mix, s0.eps_rf, s0.box, s0.rc, s0.rc_lj, periodic,
+                   cell_mode, precision)
+        self.set_replicas([s.box for s in systems], [s.n for s in systems], s0.rc, s0.rc_lj,
+                          periodic, cell_mode)
+        cat = lambda k: np.concatenate([np.asarray(getattr(s, k)) for s in systems])
+        self.upload(cat("r"), cat("v"), cat("q"), cat("L"), cat("cid"))
         return self
 
     def set_precision(self, p):
@@ -174,9 +211,10 @@ class Context:
         self._chk(self.L.mdx_update_container(self.h), "mdx_update_container")
 
     def traverse(self):
-        mac = np.zeros(4)
+        """macro sums [4] ([nrep, 4] for batched replicas)."""
+        mac = np.zeros((self.nrep, 4))
         self._chk(self.L.mdx_traverse(self.h, _ptr(mac)), "mdx_traverse")
-        return mac
+        return mac[0] if self.nrep == 1 else mac
 
     def forces(self):
         """update_container + traverse, then the molecule outputs."""
@@ -190,15 +228,18 @@ class Context:
         self._chk(self.L.mdx_leapfrog_pre(self.h, float(dt)), "mdx_leapfrog_pre")
 
     def leapfrog_post(self, dt):
-        sums = np.zeros(4)
+        sums = np.zeros((self.nrep, 4))
         self._chk(self.L.mdx_leapfrog_post(self.h, float(dt), _ptr(sums)), "mdx_leapfrog_post")
-        return sums
+        return sums[0] if self.nrep == 1 else sums
 
     def step(self, nsteps, dt, use_graph=False):
-        per = np.zeros(6 * max(nsteps, 1))
+        """Per-step records [nsteps, 6] ([nsteps, nrep, 6] for batched replicas):
+        Upot, virial, sum m v^2, sum I w^2, Upot_LJ, Upot_rest."""
+        per = np.zeros(6 * self.nrep * max(nsteps, 1))
         self._chk(self.L.mdx_step(self.h, int(nsteps), float(dt), _ptr(per), int(bool(use_graph))),
                   "mdx_step")
-        return per.reshape(-1, 6)[:nsteps]
+        per = per.reshape(-1, self.nrep, 6)[:nsteps]
+        return per[:, 0] if self.nrep == 1 else per
 
     def synchronize(self):
         self._chk(self.L.mdx_synchronize(self.h), "mdx_synchronize")

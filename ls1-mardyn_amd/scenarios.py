@@ -23,6 +23,10 @@ Workloads (reduced units, sigma = eps = 1):
 * ``binary_lj``    60/40 one-centre LJ mixture, eps22 = 0.85, sigma22 = 1.1,
                    seeded Lorentz-Berthelot factors xi, eta in [0.95, 1.05],
                    rho 0.7, T 0.9, rc 2.5.
+* ``replicas``     a state-point sweep of independent periodic LJTS systems
+                   (rc 2.5): molecule counts log-uniform in [2,000, 40,000],
+                   rho uniform in [0.05, 0.85], T uniform in [0.7, 1.3], each
+                   drawn from the seed; 512 replicas by default.
 * ``small_lj`` / ``small_mixture``: seconds-sized periodic systems for parity.
 """
 from __future__ import annotations
@@ -190,11 +194,11 @@ def fcc_lattice(ncell, a):
 
 # ------------------------------------------------------------- workloads
 
-def lj_fcc(ncell=20, rho=0.6, T=0.7, rc=2.5, seed=1):
+def lj_fcc(ncell=20, rho=0.6, T=0.7, rc=2.5, seed=1, shifted=False):
     a = (4.0 / rho) ** (1.0 / 3.0)
     r = fcc_lattice(ncell, a)
     n = len(r)
-    desc = np.array(comp_1clj(rc_shift=rc, shifted=False), np.float64)
+    desc = np.array(comp_1clj(rc_shift=rc, shifted=shifted), np.float64)
     comps = parse_components(desc)
     rng = np.random.default_rng(seed)
     cid = np.zeros(n, np.int32)
@@ -254,6 +258,29 @@ def binary_lj(n=500_000, rho=0.7, T=0.9, rc=2.5, seed=4, min_sep=0.85):
     v, L = maxwell_boltzmann(rng, cid, comps, T, q)
     return System("binary_lj", 2, desc, np.array([[xi, eta]]), 1.0, rc, rc, box, r, v, q, L * 0.0,
                   cid, T)
+
+
+def ljts_box(n, rho, T, rc=2.5, seed=0, min_sep=0.8, name="ljts_box"):
+    """One periodic truncated-shifted LJ system at density rho, temperature T."""
+    box = np.full(3, (n / rho) ** (1.0 / 3.0))
+    r = rsa_positions(seed, [((0, 0, 0), tuple(box), n)], box, min_sep)
+    desc = np.array(comp_1clj(rc_shift=rc, shifted=True), np.float64)
+    comps = parse_components(desc)
+    rng = np.random.default_rng(seed + 1000)
+    cid = np.zeros(n, np.int32)
+    q = np.tile([1.0, 0.0, 0.0, 0.0], (n, 1))
+    v, L = maxwell_boltzmann(rng, cid, comps, T, q)
+    return System(name, 1, desc, np.zeros((0, 2)), 1.0, rc, rc, box, r, v, q, L * 0.0, cid, T)
+
+
+def replicas(nrep=512, nmin=2000, nmax=40000, seed=5, rc=2.5):
+    """Independent LJTS systems of a state-point sweep (see the module notes)."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), nrep)).astype(np.int64)
+    rhos = rng.uniform(0.05, 0.85, nrep)
+    Ts = rng.uniform(0.7, 1.3, nrep)
+    return [ljts_box(int(ns[k]), float(rhos[k]), float(Ts[k]), rc, seed=10_000 + 7 * k + seed,
+                     name=f"replica{k}") for k in range(nrep)]
 
 
 def small_lj(n=400, rho=0.6, T=0.7, rc=2.5, seed=11, shifted=False):
