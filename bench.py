@@ -238,6 +238,36 @@ def run_replicas(args):
     return line
 
 
+def run_adjoint(args):
+    """Differentiation workload: binary LJ mixture, 500,000 molecules, FP64:
+    forward traversal, then the adjoint (6 eps/sigma pair derivatives of
+    Upot and virial + the force VJP for a seeded Gaussian cotangent)."""
+    s = scenarios.binary_lj(n=int(500_000 * args.scale ** 3))
+    cot = np.random.default_rng(9).normal(size=(s.n, 3))
+    with Context(0) as ctx:
+        ctx.setup_system(s, periodic=True, precision=PREC_FP64)
+        ctx.update_container()
+        ctx.traverse()
+        for _ in range(args.warmup):
+            ctx.adjoint(cot)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.traverse()
+        fwd = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.adjoint(cot)
+        adj = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.adjoint(None)
+        adj_p = (time.perf_counter() - t0) / args.steps
+    return {"workload": "binary_lj adjoint", "molecules": int(s.n),
+            "forward_traverse_ms": fwd * 1e3, "adjoint_with_vjp_ms": adj * 1e3,
+            "adjoint_params_only_ms": adj_p * 1e3,
+            "note": "host-call wall time incl. cotangent upload and result download"}
+
+
 def rel_rms(a, b):
     den = float((b ** 2).sum(1).mean())
     return float(np.sqrt(((a - b) ** 2).sum(1).mean() / den)) if den > 0 else None
@@ -339,6 +369,9 @@ def main():
               "(every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "adjoint":
+        print(json.dumps(run_adjoint(args)), flush=True)
+        return
     if args.workload == "replicas":
         print(json.dumps(run_replicas(args)), flush=True)
         return

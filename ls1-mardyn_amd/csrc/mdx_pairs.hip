@@ -28,17 +28,16 @@ template <typename R> __device__ __forceinline__ V3<R> v3(const Q4<R> &q) { retu
 
 // One site pair seen from site i (tile entry s): force on i, virial
 // (m_i - m_j) * f_i, torque on i; roles swapped where the reference swaps.
+// pj = T.p[s]; mj = T.m[s] where the tile keeps molecule positions (read
+// ahead by the caller).
 template <int TI, int TJ, typename R, bool MOLSITE, bool MULTI>
 __device__ __forceinline__ void eval_pair(const PairArgs &a, const ISite<R> &I,
-                                          const Tile<TI, R, MOLSITE, MULTI> &T, int s, R eps24,
-                                          R sig2, R shift6, Acc &acc) {
-    const Q4<R> pj = T.p[s];
+                                          const Tile<TI, R, MOLSITE, MULTI> &T, int s,
+                                          const Q4<R> &pj, const Q4<R> &mj, R eps24, R sig2,
+                                          R shift6, Acc &acc) {
     V3<R> d = {I.pos.x - pj.x, I.pos.y - pj.y, I.pos.z - pj.z};
     V3<R> dm = d;
-    if constexpr (!MOLSITE) {
-        const Q4<R> mj = T.m[s];
-        dm = {I.mol.x - mj.x, I.mol.y - mj.y, I.mol.z - mj.z};
-    }
+    if constexpr (!MOLSITE) dm = {I.mol.x - mj.x, I.mol.y - mj.y, I.mol.z - mj.z};
     PO<R> o;
     o.rf = R(0);
     R sign = R(1);
@@ -136,6 +135,9 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
     const SiteArrays &J = a.s[TJ];
     const double rc2 = (TI == T_LJ) ? a.rclj2 : a.rc2;
     const R rc2r = (R)rc2;
+    // FP32 re-check band around rc^2 (relative 1e-5, far above FP32 rounding
+    // of cell-relative coordinates)
+    const R lo = rc2r * R(1 - 1e-5), hi = rc2r * R(1 + 1e-5);
     unsigned long long nhit = 0;
     int nk = 0;
     if (lane < 27) {
@@ -159,7 +161,6 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
     if (lane == 0) T.pre[0] = 0;
     const int total = __shfl(incl, 26, 64);
     __syncthreads();
-    const int own_b = T.pre[13], own_e = T.pre[14];
     for (int base = 0; base < total; base += C::CAP) {
         const int nst = min(C::CAP, total - base);
         for (int s = lane; s < nst; s += 64) {
@@ -186,25 +187,61 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             }
         }
         __syncthreads();
-        const int sb = own_b - base, se = own_e - base;  // own cell inside this chunk
-        int cnt = 0, ub = 0;
+        int cnt = 0;
+        // Stage B. A listed candidate is final unless its molecule distance
+        // is 0 (the molecule itself, SingleCellPolicy_) or, in FP32, lies in
+        // the band [lo, hi] around rc^2, where the decision is re-made in FP64
+        // from the unrounded positions so the pair set equals the FP64 one.
+        auto accept = [&](int s, const Q4<R> &q) -> bool {
+            const V3<R> mi = C::MOL ? I.mol : I.pos;
+            const R dx = mi.x - q.x, dy = mi.y - q.y, dz = mi.z - q.z;
+            const R r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == R(0)) return false;
+            if constexpr (sizeof(R) == 4) {
+                if (r2 >= lo) {
+                    const int idx = base + s;
+                    const int k = tile_k(T.pre, idx);
+                    const int j = T.j0[k] + (idx - T.pre[k]);
+                    const double ex = I.mx - (J.mx[j] + T.shift[k][0]);
+                    const double ey = I.my - (J.my[j] + T.shift[k][1]);
+                    const double ez = I.mz - (J.mz[j] + T.shift[k][2]);
+                    const double r2d = ex * ex + ey * ey + ez * ez;
+                    return r2d < rc2 && r2d != 0.0;
+                }
+            }
+            return true;
+        };
+        // the next entry's index and cutoff record are read one iteration
+        // ahead, so their LDS latency overlaps the current body
         auto drain = [&]() {
+            int sn = 0;
+            Q4<R> qn{};
+            if (cnt > 0) {
+                sn = (int)T.lst[0][lane];
+                qn = C::MOL ? T.m[sn] : T.p[sn];
+            }
             for (int e = 0;; ++e) {
-                bool act = e < cnt;
+                const bool act = e < cnt;
                 if (!__any(act)) break;
-                if (act) {
-                    eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, (int)T.lst[e][lane], eps24, sig2,
+                const int sj = sn;
+                const Q4<R> qj = qn;
+                if (e + 1 < cnt) {
+                    sn = (int)T.lst[e + 1][lane];
+                    qn = C::MOL ? T.m[sn] : T.p[sn];
+                }
+                if (act && accept(sj, qj)) {
+                    const Q4<R> pj = C::MOL ? T.p[sj] : qj;
+                    eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj, pj, qj, eps24, sig2,
                                                          shift6, acc);
                     if (COUNT) ++nhit;
                 }
             }
             cnt = 0;
-            ub = 0;
         };
         const int per = (nst + P - 1) / P;
         for (int m0 = 0; m0 < per; m0 += kUnroll) {
-            if (ub + kUnroll > kList) drain();
-            ub += kUnroll;
+            // drain when some lane's list could overflow in this round
+            if (__any(cnt > kList - kUnroll)) drain();
             R r2v[kUnroll];
             int sv[kUnroll];
 #pragma unroll
@@ -217,30 +254,14 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                 const R dx = mi.x - q.x, dy = mi.y - q.y, dz = mi.z - q.z;
                 r2v[u] = dx * dx + dy * dy + dz * dz;
             }
+            // Stage A: keep candidates below the cutoff (FP32: up to the top
+            // of the re-check band)
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
                 const int s = sv[u];
-                if (!(valid && s < nst)) continue;
-                const R r2 = r2v[u];
-                bool hit = r2 < rc2r;
-                const bool own = s >= sb && s < se;
-                if constexpr (sizeof(R) == 4) {
-                    if (fabsf(r2 - rc2r) <= 1e-5f * rc2r) {
-                        // re-make the decision in FP64 from the unrounded positions
-                        const int idx = base + s;
-                        const int k = tile_k(T.pre, idx);
-                        const int j = T.j0[k] + (idx - T.pre[k]);
-                        double ex = I.mx - (J.mx[j] + T.shift[k][0]);
-                        double ey = I.my - (J.my[j] + T.shift[k][1]);
-                        double ez = I.mz - (J.mz[j] + T.shift[k][2]);
-                        double r2d = ex * ex + ey * ey + ez * ez;
-                        hit = r2d < rc2;
-                        if (own && r2d == 0.0) hit = false;
-                    }
-                }
-                if (own && r2 == R(0)) hit = false;
-                if (COUNT) ++ntest;
-                if (hit) {
+                const bool in = valid && s < nst;
+                if (COUNT && in) ++ntest;
+                if (in && (sizeof(R) == 4 ? r2v[u] <= hi : r2v[u] < rc2r)) {
                     T.lst[cnt][lane] = (unsigned short)s;
                     ++cnt;
                 }
@@ -280,13 +301,13 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             }
         }
         for (int chunk = i_begin; chunk < i_end; chunk += 64) {
+            // lane = ph * nI + il: P = 64 / nI phases of the chunk's nI sites
+            // split the partner candidates; lanes past P * nI idle
             const int nI = min(64, i_end - chunk);
-            int lg = 0;
-            while ((1 << lg) < nI) ++lg;
-            const int Ipad = 1 << lg, P = 64 >> lg;
-            const int il = lane & (Ipad - 1), ph = lane >> lg;
-            const bool valid = il < nI;
-            const int i = chunk + (valid ? il : 0);
+            const int P = 64 / nI;
+            const int ph = lane / nI, il = lane - ph * nI;
+            const bool valid = ph < P;
+            const int i = chunk + il;
             ISite<R> I;
             I.pos = SiteView<R>::pos(S, i);
             I.mol = MOLSITE ? I.pos : SiteView<R>::mol(S, i);
@@ -319,20 +340,32 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
                                                                     lane, T, acc, eps24, sig2,
                                                                     shift6, ntest);
             }
-            for (int o = Ipad; o < 64; o <<= 1) {
-                for (int k = 0; k < 3; ++k) {
-                    acc.f[k] += shfl_xor_d(acc.f[k], o);
-                    acc.v[k] += shfl_xor_d(acc.v[k], o);
-                    if constexpr (TI >= T_D) acc.m[k] += shfl_xor_d(acc.m[k], o);
-                }
+            // fold the phases of each site in a fixed order through LDS (the
+            // tile is free after the last pass's barrier)
+            constexpr int W = TI >= T_D ? 9 : 6;
+            static_assert(sizeof(Tile<TI, R, MOLSITE, MULTI>) >= 64 * W * sizeof(double),
+                          "tile too small for the phase fold");
+            double *fold = reinterpret_cast<double *>(&T);
+            for (int k = 0; k < 3; ++k) {
+                fold[k * 64 + lane] = valid ? acc.f[k] : 0.0;
+                fold[(3 + k) * 64 + lane] = valid ? acc.v[k] : 0.0;
+                if constexpr (TI >= T_D) fold[(6 + k) * 64 + lane] = valid ? acc.m[k] : 0.0;
             }
-            if (valid && ph == 0) {
-                S.Fx[i] = acc.f[0]; S.Fy[i] = acc.f[1]; S.Fz[i] = acc.f[2];
-                S.Vx[i] = acc.v[0]; S.Vy[i] = acc.v[1]; S.Vz[i] = acc.v[2];
+            __syncthreads();
+            if (ph == 0) {
+                double sum[W];
+                for (int k = 0; k < W; ++k) {
+                    double x = 0.0;
+                    for (int p2 = 0; p2 < P; ++p2) x += fold[k * 64 + p2 * nI + il];
+                    sum[k] = x;
+                }
+                S.Fx[i] = sum[0]; S.Fy[i] = sum[1]; S.Fz[i] = sum[2];
+                S.Vx[i] = sum[3]; S.Vy[i] = sum[4]; S.Vz[i] = sum[5];
                 if constexpr (TI >= T_D) {
-                    S.Mx[i] = acc.m[0]; S.My[i] = acc.m[1]; S.Mz[i] = acc.m[2];
+                    S.Mx[i] = sum[6]; S.My[i] = sum[7]; S.Mz[i] = sum[8];
                 }
             }
+            __syncthreads();
             wmac[0] += acc.u;
             wmac[1] += acc.vir;
             wmac[2] += acc.rf;
