@@ -229,6 +229,12 @@ int parse_model(mdx_context *c, int ncomp, const double *desc, int len) {
         }
         p += 3;
         cp.rotdof = rotdof;
+        cp.point = cp.n[2] == 0 && cp.n[3] == 0 && cp.invI[0] == 0. && cp.invI[1] == 0. &&
+                   cp.invI[2] == 0.;
+        for (int t = 0; t < 2; ++t)
+            for (int k = 0; k < cp.n[t]; ++k)
+                for (int d = 0; d < 3; ++d)
+                    if (cp.body[t][k][d] != 0.) cp.point = 0;
         cp.lj_base = base;
         base += cp.n[0];
         // massless components (only dipoles / quadrupoles) are valid for the
@@ -811,6 +817,8 @@ int mdx_set_domain(mdx_context *c, const double *box, double rc, double rc_lj, i
     c->cell_off = {0, c->ncell};
     c->rep_n.clear();
     drop_graphs(c);
+    c->cap_steps = 0;
+    TRY(ensure_steps(c, 1));
     set_cutoff_terms(c);
     TRY(upload_reps(c));
     build_tables(c);
@@ -847,6 +855,8 @@ int mdx_set_replicas(mdx_context *c, int nrep, const double *boxes, const int64_
     c->nrep = nrep;
     c->rep_n.assign(counts, counts + nrep);
     drop_graphs(c);
+    c->cap_steps = 0;  // step records are [step][nrep][6]: re-size for nrep
+    TRY(ensure_steps(c, 1));
     set_cutoff_terms(c);
     build_tables(c);
     TRY(upload_tables(c));

@@ -29,6 +29,8 @@ struct Component {
     int n[kTypes];          // nlj, nc, nd, nq
     int lj_base;            // first LJ lookup id (EnsembleBase.cpp:98-111)
     int rotdof;
+    int point;              // all sites at the centre, no D/Q sites, invI = 0: the
+                            // orientation never changes and carries no force
     double m, I[3], invI[3];
     double body[kTypes][kMaxSite][3];   // site offsets, principal frame
     double e[kTypes][kMaxSite][3];      // D, Q axes (body frame, as given)

@@ -108,6 +108,34 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // --------------------------------------------------------------- Leapfrog
 
+// Rotational half of FullMolecule::upd_preF (FullMolecule.cpp:340-360):
+// L += dt/2 M, q advanced with the half-step angular velocity, normalised.
+// Point molecules skip it: their invI and torque are 0, so L is unchanged and
+// q only re-normalised (which k_build_sites does every step, as ls1's
+// setupSoACache does).
+__device__ __forceinline__ void rotate_pre(MolArrays &s, const double *__restrict__ M,
+                                           const Component &c, double dt, int n, int i) {
+    const double dt_halve = .5 * dt;
+    double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
+    double L[3] = {s.Lx[i], s.Ly[i], s.Lz[i]};
+    double w[3], qh[4], qi[4];
+    qrotinv(q, L, w);
+    for (int d = 0; d < 3; ++d) w[d] *= c.invI[d];
+    qdiff(q, w, qh);
+    for (int k = 0; k < 4; ++k) qh[k] = qh[k] * dt_halve + q[k];
+    double qc = 1. / sqrt(qh[0] * qh[0] + qh[1] * qh[1] + qh[2] * qh[2] + qh[3] * qh[3]);
+    for (int k = 0; k < 4; ++k) qh[k] *= qc;
+    for (int d = 0; d < 3; ++d) L[d] += dt_halve * M[d * n + i];
+    qrotinv(qh, L, w);
+    for (int d = 0; d < 3; ++d) w[d] *= c.invI[d];
+    qdiff(qh, w, qi);
+    for (int k = 0; k < 4; ++k) q[k] += qi[k] * dt;
+    qc = 1. / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int k = 0; k < 4; ++k) q[k] *= qc;
+    s.qw[i] = q[0]; s.qx[i] = q[1]; s.qy[i] = q[2]; s.qz[i] = q[3];
+    s.Lx[i] = L[0]; s.Ly[i] = L[1]; s.Lz[i] = L[2];
+}
+
 // FullMolecule::upd_preF (FullMolecule.cpp:331-361), then the periodic wrap of
 // DomainDecompBase::handleDomainLeavingParticles (DomainDecompBase.cpp:199-249).
 __global__ __launch_bounds__(256) void k_leapfrog_pre(MolArrays s, const double *__restrict__ F,
@@ -126,22 +154,7 @@ __global__ __launch_bounds__(256) void k_leapfrog_pre(MolArrays s, const double 
         v[d] += dtInv2m * F[d * n + i];
         r[d] += dt * v[d];
     }
-    double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
-    double L[3] = {s.Lx[i], s.Ly[i], s.Lz[i]};
-    double w[3], qh[4], qi[4];
-    qrotinv(q, L, w);
-    for (int d = 0; d < 3; ++d) w[d] *= c.invI[d];
-    qdiff(q, w, qh);
-    for (int k = 0; k < 4; ++k) qh[k] = qh[k] * dt_halve + q[k];
-    double qc = 1. / sqrt(qh[0] * qh[0] + qh[1] * qh[1] + qh[2] * qh[2] + qh[3] * qh[3]);
-    for (int k = 0; k < 4; ++k) qh[k] *= qc;
-    for (int d = 0; d < 3; ++d) L[d] += dt_halve * M[d * n + i];
-    qrotinv(qh, L, w);
-    for (int d = 0; d < 3; ++d) w[d] *= c.invI[d];
-    qdiff(qh, w, qi);
-    for (int k = 0; k < 4; ++k) q[k] += qi[k] * dt;
-    qc = 1. / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-    for (int k = 0; k < 4; ++k) q[k] *= qc;
+    if (!c.point) rotate_pre(s, M, c, dt, n, i);
     if (g.periodic) {
         for (int d = 0; d < 3; ++d) {
             double x = r[d];
@@ -157,8 +170,6 @@ __global__ __launch_bounds__(256) void k_leapfrog_pre(MolArrays s, const double 
     }
     s.rx[i] = r[0]; s.ry[i] = r[1]; s.rz[i] = r[2];
     s.vx[i] = v[0]; s.vy[i] = v[1]; s.vz[i] = v[2];
-    s.qw[i] = q[0]; s.qx[i] = q[1]; s.qy[i] = q[2]; s.qz[i] = q[3];
-    s.Lx[i] = L[0]; s.Ly[i] = L[1]; s.Lz[i] = L[2];
 }
 
 // FullMolecule::upd_postF (FullMolecule.cpp:363-387) with per-block FP64
@@ -180,25 +191,27 @@ __global__ __launch_bounds__(256) void k_leapfrog_post(MolArrays s, const double
         double dt_halve = 0.5 * dt;
         double dtInv2m = dt_halve / c.m;
         double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
-        double L[3] = {s.Lx[i], s.Ly[i], s.Lz[i]};
         double v2 = 0.;
         for (int d = 0; d < 3; ++d) {
             v[d] += dtInv2m * F[d * n + i];
             v2 += v[d] * v[d];
-            L[d] += dt_halve * M[d * n + i];
         }
         mv2 = c.m * v2;
-        double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
-        double w[3];
-        qrotinv(q, L, w);
-        for (int d = 0; d < 3; ++d) {
-            w[d] *= c.invI[d];
-            iw2 += c.I[d] * w[d] * w[d];
+        if (!c.point) {  // point molecules: L unchanged, I w^2 = 0
+            double L[3] = {s.Lx[i], s.Ly[i], s.Lz[i]};
+            for (int d = 0; d < 3; ++d) L[d] += dt_halve * M[d * n + i];
+            double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
+            double w[3];
+            qrotinv(q, L, w);
+            for (int d = 0; d < 3; ++d) {
+                w[d] *= c.invI[d];
+                iw2 += c.I[d] * w[d] * w[d];
+            }
+            s.Lx[i] = L[0]; s.Ly[i] = L[1]; s.Lz[i] = L[2];
         }
         cnt = 1.0;
         rot = c.rotdof;
         s.vx[i] = v[0]; s.vy[i] = v[1]; s.vz[i] = v[2];
-        s.Lx[i] = L[0]; s.Ly[i] = L[1]; s.Lz[i] = L[2];
     }
     mv2 = wave_sum(mv2);
     iw2 = wave_sum(iw2);
@@ -449,7 +462,9 @@ __global__ __launch_bounds__(256) void k_calc_fm(int n, MolArrays s,
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Component &c = comps[s.cid[i]];
-    double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
+    // point molecules: every site offset is 0, so no d x F term needs q
+    double q[4] = {1, 0, 0, 0};
+    if (!c.point) q[0] = s.qw[i], q[1] = s.qx[i], q[2] = s.qy[i], q[3] = s.qz[i];
     double f[3] = {0, 0, 0}, m[3] = {0, 0, 0}, tm[3] = {0, 0, 0}, tv[3] = {0, 0, 0};
     SiteArrays *S[4] = {&s0, &s1, &s2, &s3};
 #pragma unroll
