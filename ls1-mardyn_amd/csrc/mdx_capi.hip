@@ -1017,7 +1017,6 @@ int mdx_download(mdx_context *c, int64_t n, double *r, double *v, double *q, dou
     const int nn = c->n;
     MolArrays &m = c->mol[c->cur];
     double *tmp = c->dl_tmp;
-    std::vector<double> hb((size_t)nn * 4);
     auto get = [&](const double *a, int width, int stride_in, double *out) -> int {
         if (!out) return MDX_OK;
         hipLaunchKernelGGL(k_scatter_orig, dim3(nblk(nn)), dim3(kBlock), 0, c->stream, nn, m.orig,
@@ -1054,12 +1053,6 @@ int mdx_download(mdx_context *c, int64_t n, double *r, double *v, double *q, dou
     TRY(get(c->Vi, 3, nn, Vi));
     if (ids)
         for (int i = 0; i < nn; ++i) ids[i] = c->ids[i];
-    (void)hb;
-    // release the scratch copy
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    (void)hipFree(tmp);
-    c->state_allocs.erase(std::remove(c->state_allocs.begin(), c->state_allocs.end(), (void *)tmp),
-                          c->state_allocs.end());
     return MDX_OK;
 }
 

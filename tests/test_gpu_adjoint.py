@@ -49,6 +49,23 @@ def test_adjoint_matches_oracle(make):
         assert np.array_equal(got[k], again[k]), k
 
 
+def test_adjoint_after_downloads():
+    """Downloads and the adjoint share the context's staging buffer: repeated
+    downloads and an adjoint after them keep working and agree."""
+    s = small_binary()
+    o = Oracle(s.ncomp, s.comp_desc, s.mix, s.eps_rf, s.rc, s.rc_lj, s.box)
+    cot = np.random.default_rng(3).normal(size=(s.n, 3))
+    ref = o.adjoint(s.r, s.q, s.cid, cot)
+    with gpu_ctx(s) as c:
+        d1 = c.download()
+        d2 = c.download()
+        got = c.adjoint(cot)
+        d3 = c.download()
+    for k in ("r", "F", "M"):
+        assert np.array_equal(d1[k], d2[k]) and np.array_equal(d1[k], d3[k]), k
+    np.testing.assert_allclose(got["vjp"], ref["vjp"], rtol=0, atol=1e-11 * np.abs(ref["vjp"]).max())
+
+
 def test_adjoint_oracle_binary_5000():
     s = scenarios.binary_lj(n=5000)
     print("case binary_lj 5000", flush=True)
