@@ -74,16 +74,24 @@ def make_workload(name, scale):
 
 
 def run_gpu(s, precision, steps, warmup, kernel_reps):
+    t_start = time.perf_counter()
+
+    def phase(msg):
+        log(f"    [{time.perf_counter() - t_start:7.2f} s] {msg}")
+
     with Context(0) as ctx:
         ctx.setup_system(s, periodic=True, precision=precision)
+        phase("uploaded")
         ctx.update_container()
         ctx.traverse()
         init = ctx.download()
+        phase("first traversal")
         # work counters of one traversal (separate, uncounted runs are timed)
         ctx.set_counting(True)
         ctx.traverse()
         counters = ctx.counters()
         ctx.set_counting(False)
+        phase("counted traversal")
         # pair-kernel duration: HIP events on the context stream around the
         # pair kernels, averaged over kernel_reps traversals
         ctx.set_timing(True)
@@ -92,9 +100,11 @@ def run_gpu(s, precision, steps, warmup, kernel_reps):
             ctx.traverse()
             pair_ms.append(ctx.last_kernel_ms()[0])
         ctx.set_timing(False)
+        phase(f"timed traversals: pair kernels {np.mean(pair_ms):.3f} ms")
         if warmup:
             ctx.step(warmup, s.dt, use_graph=True)
         ctx.synchronize()
+        phase("warm-up steps")
         try:
             import torch
 

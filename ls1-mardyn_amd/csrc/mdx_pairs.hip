@@ -135,6 +135,9 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
     const SiteArrays &J = a.s[TJ];
     const double rc2 = (TI == T_LJ) ? a.rclj2 : a.rc2;
     const R rc2r = (R)rc2;
+    // idle lanes test from far away: r^2 overflows or exceeds any cutoff
+    const V3<R> far = {R(1e30), R(1e30), R(1e30)};
+    const V3<R> Imol = valid ? I.mol : far, Ipos = valid ? I.pos : far;
     // FP32 re-check band around rc^2 (relative 1e-5, far above FP32 rounding
     // of cell-relative coordinates)
     const R lo = rc2r * R(1 - 1e-5), hi = rc2r * R(1 + 1e-5);
@@ -238,33 +241,47 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             }
             cnt = 0;
         };
-        const int per = (nst + P - 1) / P;
-        for (int m0 = 0; m0 < per; m0 += kUnroll) {
+        // Stage A: keep candidates below the cutoff (FP32: up to the top of
+        // the re-check band). Row m holds candidates m * P + ph; rows below
+        // full are inside [0, nst) for every phase and need no bounds test,
+        // and idle lanes carry a far-away position, so the bulk loop is a
+        // read, a distance and a compare per candidate.
+        const int per = (nst + P - 1) / P, full = per - 1;
+        const Q4<R> *rec = (C::MOL ? T.m : T.p) + ph;
+        const V3<R> mi = C::MOL ? Imol : Ipos;
+        const R cut = sizeof(R) == 4 ? hi : rc2r;
+        auto test = [&](R r2) { return sizeof(R) == 4 ? r2 <= cut : r2 < cut; };
+        int m0 = 0;
+        for (; m0 + kUnroll <= full; m0 += kUnroll) {
             // drain when some lane's list could overflow in this round
             if (__any(cnt > kList - kUnroll)) drain();
             R r2v[kUnroll];
-            int sv[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
-                const int s = ph + (m0 + u) * P;
-                sv[u] = s;
-                const int sc = min(s, nst - 1);
-                const Q4<R> q = C::MOL ? T.m[sc] : T.p[sc];
-                const V3<R> mi = C::MOL ? I.mol : I.pos;
+                const Q4<R> q = rec[(m0 + u) * P];
                 const R dx = mi.x - q.x, dy = mi.y - q.y, dz = mi.z - q.z;
                 r2v[u] = dx * dx + dy * dy + dz * dz;
             }
-            // Stage A: keep candidates below the cutoff (FP32: up to the top
-            // of the re-check band)
+            if (COUNT && valid) ntest += kUnroll;
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
-                const int s = sv[u];
-                const bool in = valid && s < nst;
-                if (COUNT && in) ++ntest;
-                if (in && (sizeof(R) == 4 ? r2v[u] <= hi : r2v[u] < rc2r)) {
-                    T.lst[cnt][lane] = (unsigned short)s;
+                if (test(r2v[u])) {
+                    T.lst[cnt][lane] = (unsigned short)(ph + (m0 + u) * P);
                     ++cnt;
                 }
+            }
+        }
+        for (; m0 < per; ++m0) {  // the last rows, bounds-tested
+            if (__any(cnt > kList - 1)) drain();
+            const int sidx = ph + m0 * P;
+            const Q4<R> q = rec[min(m0 * P, nst - 1 - ph)];
+            const R dx = mi.x - q.x, dy = mi.y - q.y, dz = mi.z - q.z;
+            const R r2 = dx * dx + dy * dy + dz * dz;
+            const bool in = valid && sidx < nst;
+            if (COUNT && in) ++ntest;
+            if (in && test(r2)) {
+                T.lst[cnt][lane] = (unsigned short)sidx;
+                ++cnt;
             }
         }
         drain();
