@@ -92,6 +92,8 @@ def run_gpu(s, precision, steps, warmup, kernel_reps):
         counters = ctx.counters()
         ctx.set_counting(False)
         phase("counted traversal")
+        shape, _ = ctx.autotune(reps=2)
+        phase(f"autotuned launch shape (cells per block, blocks per XCD) = {shape}")
         # pair-kernel duration: HIP events on the context stream around the
         # pair kernels, averaged over kernel_reps traversals
         ctx.set_timing(True)
@@ -117,7 +119,7 @@ def run_gpu(s, precision, steps, warmup, kernel_reps):
         ctx.synchronize()
         t1 = time.perf_counter()
     return dict(seconds=t1 - t0, per_step=per, counters=counters, pair_ms=float(np.mean(pair_ms)),
-                pair_ms_all=pair_ms, init=init)
+                pair_ms_all=pair_ms, init=init, shape=shape)
 
 
 def cpu_baseline(workload, budget_s):
@@ -429,6 +431,7 @@ def main():
                      "traffic": measured_traffic(args.workload, args.scale, main_tag),
                      "kernel": "k_pairs<LJ>", "flops_per_launch": flops,
                      "kernel_ms": r["pair_ms"]},
+        "launch_shape": {"cells_per_block": r["shape"][0], "blocks_per_xcd": r["shape"][1]},
     }
     if "fp64" in res and "fp32" in res:
         r64 = res["fp64"]

@@ -265,7 +265,7 @@ template <typename R> struct ISite {
     V3<R> pos, mol, e;
     R par;
     int ljid;
-    double mx, my, mz;  // FP64 molecule position for the cutoff re-check
+    int idx;            // site index (FP64 molecule position for the cutoff re-check)
 };
 
 template <int TI, int TJ> struct Kind;  // 0 LJ,1 CC,2 CD,3 DD,4 CQ,5 DQ,6 QQ

@@ -96,7 +96,6 @@ __device__ __forceinline__ void eval_pair(const PairArgs &a, const ISite<R> &I,
     acc.v[0] += vx;
     acc.v[1] += vy;
     acc.v[2] += vz;
-    acc.vir += vx + vy + vz;
     acc.u += (double)o.u;
     if constexpr (TI == T_D && TJ == T_D) acc.rf += (double)o.rf;
     if constexpr (TI >= T_D) {
@@ -205,9 +204,10 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                     const int idx = base + s;
                     const int k = tile_k(T.pre, idx);
                     const int j = T.j0[k] + (idx - T.pre[k]);
-                    const double ex = I.mx - (J.mx[j] + T.shift[k][0]);
-                    const double ey = I.my - (J.my[j] + T.shift[k][1]);
-                    const double ez = I.mz - (J.mz[j] + T.shift[k][2]);
+                    const SiteArrays &S = a.s[TI];
+                    const double ex = S.mx[I.idx] - (J.mx[j] + T.shift[k][0]);
+                    const double ey = S.my[I.idx] - (J.my[j] + T.shift[k][1]);
+                    const double ez = S.mz[I.idx] - (J.mz[j] + T.shift[k][2]);
                     const double r2d = ex * ex + ey * ey + ez * ez;
                     return r2d < rc2 && r2d != 0.0;
                 }
@@ -328,9 +328,7 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             ISite<R> I;
             I.pos = SiteView<R>::pos(S, i);
             I.mol = MOLSITE ? I.pos : SiteView<R>::mol(S, i);
-            I.mx = S.mx[i];
-            I.my = S.my[i];
-            I.mz = S.mz[i];
+            I.idx = i;
             if constexpr (TI >= T_D) I.e = SiteView<R>::ori(S, i);
             if constexpr (TI == T_LJ)
                 I.ljid = S.ljid[i];
@@ -384,7 +382,7 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             }
             __syncthreads();
             wmac[0] += acc.u;
-            wmac[1] += acc.vir;
+            wmac[1] += acc.v[0] + acc.v[1] + acc.v[2];  // virial = trace of the site tensors
             wmac[2] += acc.rf;
         }
     }
