@@ -318,13 +318,15 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             }
         }
         for (int chunk = i_begin; chunk < i_end; chunk += 64) {
-            // lane = ph * nI + il: P = 64 / nI phases of the chunk's nI sites
-            // split the partner candidates; lanes past P * nI idle
+            // lane = ph * Ipad + il with Ipad = pow2ceil(nI): P = 64 / Ipad
+            // phases of the chunk's nI sites split the partner candidates
             const int nI = min(64, i_end - chunk);
-            const int P = 64 / nI;
-            const int ph = lane / nI, il = lane - ph * nI;
-            const bool valid = ph < P;
-            const int i = chunk + il;
+            int lg = 0;
+            while ((1 << lg) < nI) ++lg;
+            const int Ipad = 1 << lg, P = 64 >> lg;
+            const int il = lane & (Ipad - 1), ph = lane >> lg;
+            const bool valid = il < nI;
+            const int i = chunk + (valid ? il : 0);
             ISite<R> I;
             I.pos = SiteView<R>::pos(S, i);
             I.mol = MOLSITE ? I.pos : SiteView<R>::mol(S, i);
@@ -355,34 +357,23 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
                                                                     lane, T, acc, eps24, sig2,
                                                                     shift6, ntest);
             }
-            // fold the phases of each site in a fixed order through LDS (the
-            // tile is free after the last pass's barrier)
-            constexpr int W = TI >= T_D ? 9 : 6;
-            static_assert(sizeof(Tile<TI, R, MOLSITE, MULTI>) >= 64 * W * sizeof(double),
-                          "tile too small for the phase fold");
-            double *fold = reinterpret_cast<double *>(&T);
-            for (int k = 0; k < 3; ++k) {
-                fold[k * 64 + lane] = valid ? acc.f[k] : 0.0;
-                fold[(3 + k) * 64 + lane] = valid ? acc.v[k] : 0.0;
-                if constexpr (TI >= T_D) fold[(6 + k) * 64 + lane] = valid ? acc.m[k] : 0.0;
-            }
-            __syncthreads();
-            if (ph == 0) {
-                double sum[W];
-                for (int k = 0; k < W; ++k) {
-                    double x = 0.0;
-                    for (int p2 = 0; p2 < P; ++p2) x += fold[k * 64 + p2 * nI + il];
-                    sum[k] = x;
-                }
-                S.Fx[i] = sum[0]; S.Fy[i] = sum[1]; S.Fz[i] = sum[2];
-                S.Vx[i] = sum[3]; S.Vy[i] = sum[4]; S.Vz[i] = sum[5];
-                if constexpr (TI >= T_D) {
-                    S.Mx[i] = sum[6]; S.My[i] = sum[7]; S.Mz[i] = sum[8];
-                }
-            }
-            __syncthreads();
-            wmac[0] += acc.u;
+            // fold the phases of each site (butterfly over the phase bits; the
+            // virial sums below take the per-lane parts before the fold)
             wmac[1] += acc.v[0] + acc.v[1] + acc.v[2];  // virial = trace of the site tensors
+            for (int o = Ipad; o < 64; o <<= 1)
+                for (int k = 0; k < 3; ++k) {
+                    acc.f[k] += shfl_xor_d(acc.f[k], o);
+                    acc.v[k] += shfl_xor_d(acc.v[k], o);
+                    if constexpr (TI >= T_D) acc.m[k] += shfl_xor_d(acc.m[k], o);
+                }
+            if (valid && ph == 0) {
+                S.Fx[i] = acc.f[0]; S.Fy[i] = acc.f[1]; S.Fz[i] = acc.f[2];
+                S.Vx[i] = acc.v[0]; S.Vy[i] = acc.v[1]; S.Vz[i] = acc.v[2];
+                if constexpr (TI >= T_D) {
+                    S.Mx[i] = acc.m[0]; S.My[i] = acc.m[1]; S.Mz[i] = acc.m[2];
+                }
+            }
+            wmac[0] += acc.u;
             wmac[2] += acc.rf;
         }
     }

@@ -19,7 +19,7 @@ Workloads (reduced units, sigma = eps = 1):
 * ``mixture``      rigid 2CLJQ (two LJ sites 0.5 apart, quadrupole Q = 2 on
                    the axis) and 4-site water-like molecules (LJ oxygen, two
                    +0.52 hydrogens, a -1.04 M site), 50/50, rho 0.5, T 1.0,
-                   rc 3, random orientations.
+                   rc 3, random orientations, FCC lattice start.
 * ``binary_lj``    60/40 one-centre LJ mixture, eps22 = 0.85, sigma22 = 1.1,
                    seeded Lorentz-Berthelot factors xi, eta in [0.95, 1.05],
                    rho 0.7, T 0.9, rc 2.5.
@@ -229,13 +229,19 @@ def ljts_slab(scale=1.0, rho_l=0.73, rho_v=0.02, T=0.8, rc=2.5, seed=2, min_sep=
                   T)
 
 
-def mixture(n=1_000_000, rho=0.5, T=1.0, rc=3.0, seed=3, min_sep=0.9):
+def mixture(n=1_000_000, rho=0.5, T=1.0, rc=3.0, seed=3):
+    """Molecules start on an FCC lattice at density rho (k^3 unit cells, the
+    smallest k with 4 k^3 >= n, random vacancies removed down to n): random
+    placement cannot avoid site overlaps of extended molecules at rho = 0.5,
+    and the overlaps blow the system up within a few steps."""
+    rng = np.random.default_rng(seed + 1000)
+    k = int(math.ceil((n / 4.0) ** (1.0 / 3.0) - 1e-9))
     Lb = (n / rho) ** (1.0 / 3.0)
     box = np.full(3, Lb)
-    r = rsa_positions(seed, [((0, 0, 0), tuple(box), n)], box, min_sep)
+    r = fcc_lattice(k, Lb / k)
+    r = r[np.sort(rng.permutation(len(r))[:n])]
     desc = np.array(comp_2cljq(rc) + comp_water(rc), np.float64)
     comps = parse_components(desc)
-    rng = np.random.default_rng(seed + 1000)
     cid = np.zeros(n, np.int32)
     cid[rng.permutation(n)[: n // 2]] = 1
     q = random_quaternions(rng, n)
