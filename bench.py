@@ -44,6 +44,9 @@ PEAK_FP64_VECTOR_TF = 78.6
 FLOPS_PER_PAIR = np.array([8, 41, 34, 62, 135, 79, 151, 161], dtype=np.float64)
 
 
+PAIR_SHAPE = None  # (cells per block, blocks per XCD) or None: autotune
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -92,8 +95,13 @@ def run_gpu(s, precision, steps, warmup, kernel_reps):
         counters = ctx.counters()
         ctx.set_counting(False)
         phase("counted traversal")
-        shape, _ = ctx.autotune(reps=2)
-        phase(f"autotuned launch shape (cells per block, blocks per XCD) = {shape}")
+        if PAIR_SHAPE is None:
+            shape, _ = ctx.autotune(reps=2)
+            phase(f"autotuned launch shape (cells per block, blocks per XCD) = {shape}")
+        else:
+            shape = PAIR_SHAPE
+            ctx.set_pair_config(*shape)
+            phase(f"fixed launch shape (cells per block, blocks per XCD) = {shape}")
         # pair-kernel duration: HIP events on the context stream around the
         # pair kernels, averaged over kernel_reps traversals
         ctx.set_timing(True)
@@ -360,7 +368,12 @@ def main():
                     help="replica count of the batched-replicas workload")
     ap.add_argument("--plain-steps", type=int, default=10,
                     help="steps of the per-replica plain-launch comparison")
+    ap.add_argument("--pair-shape", default=None, metavar="CPB,BPX",
+                    help="fixed pair-kernel launch shape instead of the autotuner (for profiling)")
     args = ap.parse_args()
+    if args.pair_shape:
+        global PAIR_SHAPE
+        PAIR_SHAPE = tuple(int(x) for x in args.pair_shape.split(","))
     if args.precision_report:
         print(json.dumps({"precision_report": precision_report(args.scale, args.drift_steps)}),
               flush=True)
