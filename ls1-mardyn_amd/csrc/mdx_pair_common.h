@@ -302,7 +302,9 @@ template <int TI, typename R, bool MOLSITE, bool MULTI> struct TileCfg {
 template <int TI, typename R, bool MOLSITE, bool MULTI> struct Tile {
     using C = TileCfg<TI, R, MOLSITE, MULTI>;
     Q4<R> p[C::CAP];                   // site position; w = charge / moment / LJ id
-    Q4<R> m[C::MOL ? C::CAP : 1];      // molecule position
+    Q4<R> m[C::MOL ? C::CAP : 1];      // molecule position; w = its site count at
+                                       // the molecule's first site, 0 elsewhere
+    unsigned short h[C::MOL ? C::CAP : 1];  // first-site (head) indices of the tile
     Q4<R> e[C::ORI ? C::CAP : 1];      // orientation (dipole / quadrupole axis)
     unsigned short lst[kList][64];     // per-lane hit lists (tile indices)
     int j0[27], pre[28];

@@ -163,9 +163,32 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
     if (lane == 0) T.pre[0] = 0;
     const int total = __shfl(incl, 26, 64);
     __syncthreads();
-    for (int base = 0; base < total; base += C::CAP) {
-        const int nst = min(C::CAP, total - base);
-        for (int s = lane; s < nst; s += 64) {
+    // Extended molecules (C::MOL) are tested once per molecule: the tile keeps
+    // the molecule's sites contiguous, the head list names each molecule's
+    // first site, stage A runs over heads and stage B over a hit's sites.
+    // Chunks therefore end at molecule boundaries.
+    auto is_head = [&](int idx) -> bool {
+        const int k = tile_k(T.pre, idx);
+        if (idx == T.pre[k]) return true;  // a cell range starts with a molecule
+        const int j = T.j0[k] + (idx - T.pre[k]);
+        return J.mol[j - 1] != J.mol[j];
+    };
+    for (int base = 0; base < total;) {
+        int nst = min(C::CAP, total - base);
+        if constexpr (C::MOL) {
+            if (base + nst < total) {  // back off to the first site of the split molecule
+                const int t = lane;
+                const bool h = t < kMaxSite && t < nst && is_head(base + nst - t);
+                const unsigned long long hm = __ballot(h);
+                if (hm) nst -= __ffsll((long long)hm) - 1;
+            }
+        }
+        int nh = nst;
+        int hb = 0;
+        for (int s0 = 0; s0 < nst; s0 += 64) {
+            const int s = s0 + lane;
+            bool head = false;
+            if (s < nst) {
             const int idx = base + s;
             const int k = tile_k(T.pre, idx);
             const int j = T.j0[k] + (idx - T.pre[k]);
@@ -181,13 +204,27 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             T.p[s] = q;
             if constexpr (C::MOL) {
                 const V3<R> m = SiteView<R>::mol(J, j);
-                T.m[s] = Q4<R>{m.x + off.x, m.y + off.y, m.z + off.z, R(0)};
+                head = idx == T.pre[k];
+                if (!head) head = J.mol[j - 1] != J.mol[j];
+                R w = R(0);
+                if (head) {
+                    const int mo = J.mol[j];
+                    w = (R)(J.mstart[mo + 1] - J.mstart[mo]);
+                }
+                T.m[s] = Q4<R>{m.x + off.x, m.y + off.y, m.z + off.z, w};
             }
             if constexpr (TI != T_LJ && TJ >= T_D) {
                 const V3<R> e = SiteView<R>::ori(J, j);
                 T.e[s] = Q4<R>{e.x, e.y, e.z, R(0)};
             }
+            }
+            if constexpr (C::MOL) {  // compact the heads in tile order
+                const unsigned long long hm = __ballot(head);
+                if (head) T.h[hb + __popcll(hm & ((1ull << lane) - 1ull))] = (unsigned short)s;
+                hb += __popcll(hm);
+            }
         }
+        if constexpr (C::MOL) nh = hb;
         __syncthreads();
         int cnt = 0;
         // Stage B. A listed candidate is final unless its molecule distance
@@ -233,10 +270,19 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                     qn = C::MOL ? T.m[sn] : T.p[sn];
                 }
                 if (act && accept(sj, qj)) {
-                    const Q4<R> pj = C::MOL ? T.p[sj] : qj;
-                    eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj, pj, qj, eps24, sig2,
-                                                         shift6, acc);
-                    if (COUNT) ++nhit;
+                    if constexpr (C::MOL) {
+                        const int ns = (int)qj.w;  // the molecule's sites sj .. sj + ns - 1
+                        for (int t = 0; t < ns; ++t) {
+                            const Q4<R> pj = T.p[sj + t];
+                            eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj + t, pj, qj, eps24,
+                                                                 sig2, shift6, acc);
+                        }
+                        if (COUNT) nhit += ns;
+                    } else {
+                        eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj, qj, qj, eps24, sig2,
+                                                             shift6, acc);
+                        if (COUNT) ++nhit;
+                    }
                 }
             }
             cnt = 0;
@@ -246,8 +292,13 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         // full are inside [0, nst) for every phase and need no bounds test,
         // and idle lanes carry a far-away position, so the bulk loop is a
         // read, a distance and a compare per candidate.
-        const int per = (nst + P - 1) / P, full = per - 1;
+        const int per = (nh + P - 1) / P, full = per - 1;
         const Q4<R> *rec = (C::MOL ? T.m : T.p) + ph;
+        // candidate c of the stage-A sequence -> tile index
+        auto cand = [&](int c) -> int {
+            if constexpr (C::MOL) return (int)T.h[c];
+            else return c;
+        };
         const V3<R> mi = C::MOL ? Imol : Ipos;
         const R cut = sizeof(R) == 4 ? hi : rc2r;
         auto test = [&](R r2) { return sizeof(R) == 4 ? r2 <= cut : r2 < cut; };
@@ -256,9 +307,17 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             // drain when some lane's list could overflow in this round
             if (__any(cnt > kList - kUnroll)) drain();
             R r2v[kUnroll];
+            int sv[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
-                const Q4<R> q = rec[(m0 + u) * P];
+                Q4<R> q;
+                if constexpr (C::MOL) {
+                    sv[u] = cand(ph + (m0 + u) * P);
+                    q = T.m[sv[u]];
+                } else {
+                    sv[u] = ph + (m0 + u) * P;
+                    q = rec[(m0 + u) * P];
+                }
                 const R dx = mi.x - q.x, dy = mi.y - q.y, dz = mi.z - q.z;
                 r2v[u] = dx * dx + dy * dy + dz * dz;
             }
@@ -266,18 +325,19 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
                 if (test(r2v[u])) {
-                    T.lst[cnt][lane] = (unsigned short)(ph + (m0 + u) * P);
+                    T.lst[cnt][lane] = (unsigned short)sv[u];
                     ++cnt;
                 }
             }
         }
         for (; m0 < per; ++m0) {  // the last rows, bounds-tested
             if (__any(cnt > kList - 1)) drain();
-            const int sidx = ph + m0 * P;
-            const Q4<R> q = rec[min(m0 * P, nst - 1 - ph)];
+            const int c = ph + m0 * P;
+            const int sidx = cand(min(c, nh - 1));
+            const Q4<R> q = C::MOL ? T.m[sidx] : T.p[sidx];
             const R dx = mi.x - q.x, dy = mi.y - q.y, dz = mi.z - q.z;
             const R r2 = dx * dx + dy * dy + dz * dz;
-            const bool in = valid && sidx < nst;
+            const bool in = valid && c < nh;
             if (COUNT && in) ++ntest;
             if (in && test(r2)) {
                 T.lst[cnt][lane] = (unsigned short)sidx;
@@ -286,6 +346,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         }
         drain();
         __syncthreads();
+        base += nst;
     }
     if (COUNT) {
         for (int o = 32; o > 0; o >>= 1) nhit += __shfl_xor(nhit, o, 64);
