@@ -163,10 +163,13 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
     if (lane == 0) T.pre[0] = 0;
     const int total = __shfl(incl, 26, 64);
     __syncthreads();
-    // Extended molecules (C::MOL) are tested once per molecule: the tile keeps
-    // the molecule's sites contiguous, the head list names each molecule's
-    // first site, stage A runs over heads and stage B over a hit's sites.
-    // Chunks therefore end at molecule boundaries.
+    // Where the j molecules carry several sites of type TJ (a.heads[TJ]), they
+    // are tested once per molecule: the tile keeps a molecule's sites
+    // contiguous, the head list names each molecule's first site, stage A runs
+    // over heads and stage B over a hit's sites; chunks then end at molecule
+    // boundaries. With about one site per molecule the head indirection costs
+    // more than it saves, and every site is a candidate.
+    const bool heads = C::MOL && a.heads[TJ];
     auto is_head = [&](int idx) -> bool {
         const int k = tile_k(T.pre, idx);
         if (idx == T.pre[k]) return true;  // a cell range starts with a molecule
@@ -175,7 +178,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
     };
     for (int base = 0; base < total;) {
         int nst = min(C::CAP, total - base);
-        if constexpr (C::MOL) {
+        if (heads) {
             if (base + nst < total) {  // back off to the first site of the split molecule
                 const int t = lane;
                 const bool h = t < kMaxSite && t < nst && is_head(base + nst - t);
@@ -204,8 +207,10 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             T.p[s] = q;
             if constexpr (C::MOL) {
                 const V3<R> m = SiteView<R>::mol(J, j);
-                head = idx == T.pre[k];
-                if (!head) head = J.mol[j - 1] != J.mol[j];
+                if (heads) {
+                    head = idx == T.pre[k];
+                    if (!head) head = J.mol[j - 1] != J.mol[j];
+                }
                 R w = R(0);
                 if (head) {
                     const int mo = J.mol[j];
@@ -218,13 +223,13 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                 T.e[s] = Q4<R>{e.x, e.y, e.z, R(0)};
             }
             }
-            if constexpr (C::MOL) {  // compact the heads in tile order
+            if (heads) {  // compact the heads in tile order
                 const unsigned long long hm = __ballot(head);
                 if (head) T.h[hb + __popcll(hm & ((1ull << lane) - 1ull))] = (unsigned short)s;
                 hb += __popcll(hm);
             }
         }
-        if constexpr (C::MOL) nh = hb;
+        if (heads) nh = hb;
         __syncthreads();
         int cnt = 0;
         // Stage B. A listed candidate is final unless its molecule distance
@@ -270,7 +275,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                     qn = C::MOL ? T.m[sn] : T.p[sn];
                 }
                 if (act && accept(sj, qj)) {
-                    if constexpr (C::MOL) {
+                    if (heads) {
                         const int ns = (int)qj.w;  // the molecule's sites sj .. sj + ns - 1
                         for (int t = 0; t < ns; ++t) {
                             const Q4<R> pj = T.p[sj + t];
@@ -279,7 +284,8 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                         }
                         if (COUNT) nhit += ns;
                     } else {
-                        eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj, qj, qj, eps24, sig2,
+                        const Q4<R> pj = C::MOL ? T.p[sj] : qj;
+                        eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj, pj, qj, eps24, sig2,
                                                              shift6, acc);
                         if (COUNT) ++nhit;
                     }
@@ -295,10 +301,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         const int per = (nh + P - 1) / P, full = per - 1;
         const Q4<R> *rec = (C::MOL ? T.m : T.p) + ph;
         // candidate c of the stage-A sequence -> tile index
-        auto cand = [&](int c) -> int {
-            if constexpr (C::MOL) return (int)T.h[c];
-            else return c;
-        };
+        auto cand = [&](int c) -> int { return heads ? (int)T.h[c] : c; };
         const V3<R> mi = C::MOL ? Imol : Ipos;
         const R cut = sizeof(R) == 4 ? hi : rc2r;
         auto test = [&](R r2) { return sizeof(R) == 4 ? r2 <= cut : r2 < cut; };
@@ -311,7 +314,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
                 Q4<R> q;
-                if constexpr (C::MOL) {
+                if (heads) {
                     sv[u] = cand(ph + (m0 + u) * P);
                     q = T.m[sv[u]];
                 } else {
