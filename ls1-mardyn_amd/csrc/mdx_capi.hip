@@ -86,7 +86,6 @@ struct mdx_context {
     // state
     int n = 0;
     int nsites[kTypes] = {0, 0, 0, 0};
-    int heads[kTypes] = {0, 0, 0, 0};  // molecule-level partner tests per type
     MolArrays mol[2]{};
     int cur = 0;
     double *F = nullptr, *M = nullptr, *Vi = nullptr;
@@ -541,7 +540,6 @@ int launch_traverse(mdx_context *c) {
     for (int t = 0; t < kTypes; ++t) {
         a.s[t] = c->sites[t];
         a.present[t] = c->nsites[t] > 0;
-        a.heads[t] = c->heads[t];
     }
     a.pp.eps24 = c->d_eps24;
     a.pp.sig2 = c->d_sig2;
@@ -655,14 +653,6 @@ int state_setup(mdx_context *c, int64_t n64, const int32_t *cid_host) {
     }
     for (int t = 0; t < kTypes; ++t)
         if (c->nsites[t] >= (1 << 27)) return fail(c, MDX_ERR_CAPACITY, "too many sites");
-    // partners of type t are tested per molecule when their molecules carry
-    // at least two type-t sites on average (measured: a win for 3-charge
-    // water, a loss for ~1.5 LJ sites per molecule)
-    for (int t = 0; t < kTypes; ++t) {
-        long mols = 0;
-        for (int i = 0; i < n; ++i) mols += c->model.comp[cid_host[i]].n[t] > 0;
-        c->heads[t] = mols > 0 && c->nsites[t] >= 2 * mols;
-    }
     for (auto &m : c->mol) TRY(alloc_mol(c, m, n));
     TRY(dalloc(c, &c->F, 3 * (size_t)n, true));
     TRY(dalloc(c, &c->M, 3 * (size_t)n, true));

@@ -112,7 +112,6 @@ struct PairArgs {
     int cpb;            // cells per work item (consecutive cells, one block)
     int items_per_xcd;  // work items of each XCD's contiguous range
     int present[kTypes];
-    int heads[kTypes];  // test type-t partners once per molecule (several t sites each)
     double *macro_part; // [ncell][4] per-cell sums {u6 (LJ), uXpoles, virial, rf}
     int accumulate;     // add to macro_part rows (a later i-site type) instead of writing
     unsigned long long *counters;  // [8] or null

@@ -304,7 +304,8 @@ template <int TI, typename R, bool MOLSITE, bool MULTI> struct Tile {
     Q4<R> p[C::CAP];                   // site position; w = charge / moment / LJ id
     Q4<R> m[C::MOL ? C::CAP : 1];      // molecule position; w = its site count at
                                        // the molecule's first site, 0 elsewhere
-    unsigned short h[C::MOL ? C::CAP : 1];  // first-site (head) indices of the tile
+    // first-site (head) indices of a tile of charges (partners tested per molecule)
+    unsigned short h[C::MOL && TI != T_LJ ? C::CAP : 1];
     Q4<R> e[C::ORI ? C::CAP : 1];      // orientation (dipole / quadrupole axis)
     unsigned short lst[kList][64];     // per-lane hit lists (tile indices)
     int j0[27], pre[28];

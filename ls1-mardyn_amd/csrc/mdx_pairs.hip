@@ -163,13 +163,14 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
     if (lane == 0) T.pre[0] = 0;
     const int total = __shfl(incl, 26, 64);
     __syncthreads();
-    // Where the j molecules carry several sites of type TJ (a.heads[TJ]), they
-    // are tested once per molecule: the tile keeps a molecule's sites
-    // contiguous, the head list names each molecule's first site, stage A runs
-    // over heads and stage B over a hit's sites; chunks then end at molecule
-    // boundaries. With about one site per molecule the head indirection costs
-    // more than it saves, and every site is a candidate.
-    const bool heads = C::MOL && a.heads[TJ];
+    // Charge partners are tested once per molecule (neutral molecules carry
+    // two or more charges): the tile keeps a molecule's sites contiguous, the
+    // head list names each molecule's first site, stage A runs over heads and
+    // stage B over a hit's sites; chunks then end at molecule boundaries. For
+    // LJ, dipole and quadrupole partners (typically one or two per molecule)
+    // the indirection costs more than it saves (measured) and every site is a
+    // candidate.
+    constexpr bool heads = C::MOL && TJ == T_C;
     auto is_head = [&](int idx) -> bool {
         const int k = tile_k(T.pre, idx);
         if (idx == T.pre[k]) return true;  // a cell range starts with a molecule
@@ -178,7 +179,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
     };
     for (int base = 0; base < total;) {
         int nst = min(C::CAP, total - base);
-        if (heads) {
+        if constexpr (heads) {
             if (base + nst < total) {  // back off to the first site of the split molecule
                 const int t = lane;
                 const bool h = t < kMaxSite && t < nst && is_head(base + nst - t);
@@ -207,7 +208,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             T.p[s] = q;
             if constexpr (C::MOL) {
                 const V3<R> m = SiteView<R>::mol(J, j);
-                if (heads) {
+                if constexpr (heads) {
                     head = idx == T.pre[k];
                     if (!head) head = J.mol[j - 1] != J.mol[j];
                 }
@@ -223,13 +224,13 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                 T.e[s] = Q4<R>{e.x, e.y, e.z, R(0)};
             }
             }
-            if (heads) {  // compact the heads in tile order
+            if constexpr (heads) {  // compact the heads in tile order
                 const unsigned long long hm = __ballot(head);
                 if (head) T.h[hb + __popcll(hm & ((1ull << lane) - 1ull))] = (unsigned short)s;
                 hb += __popcll(hm);
             }
         }
-        if (heads) nh = hb;
+        if constexpr (heads) nh = hb;
         __syncthreads();
         int cnt = 0;
         // Stage B. A listed candidate is final unless its molecule distance
@@ -275,7 +276,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                     qn = C::MOL ? T.m[sn] : T.p[sn];
                 }
                 if (act && accept(sj, qj)) {
-                    if (heads) {
+                    if constexpr (heads) {
                         const int ns = (int)qj.w;  // the molecule's sites sj .. sj + ns - 1
                         for (int t = 0; t < ns; ++t) {
                             const Q4<R> pj = T.p[sj + t];
@@ -301,7 +302,10 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         const int per = (nh + P - 1) / P, full = per - 1;
         const Q4<R> *rec = (C::MOL ? T.m : T.p) + ph;
         // candidate c of the stage-A sequence -> tile index
-        auto cand = [&](int c) -> int { return heads ? (int)T.h[c] : c; };
+        auto cand = [&](int c) -> int {
+            if constexpr (heads) return (int)T.h[c];
+            else return c;
+        };
         const V3<R> mi = C::MOL ? Imol : Ipos;
         const R cut = sizeof(R) == 4 ? hi : rc2r;
         auto test = [&](R r2) { return sizeof(R) == 4 ? r2 <= cut : r2 < cut; };
@@ -314,7 +318,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
                 Q4<R> q;
-                if (heads) {
+                if constexpr (heads) {
                     sv[u] = cand(ph + (m0 + u) * P);
                     q = T.m[sv[u]];
                 } else {
