@@ -285,7 +285,9 @@ template <> struct Kind<T_Q, T_Q> { static constexpr int k = 6; };
 // reference's halo copy r + L; FP32: + offset of the neighbour's corner).
 // One 16-byte (FP32) or 32-byte (FP64) record per site and field, so a site
 // is fetched with one or two ds_read_b128.
-constexpr int kList = 24;  // per-lane hit list entries
+// per-lane hit list entries: 24 for FP32, 16 for FP64 (measured best; the
+// shorter FP64 list keeps its larger tile within the LDS budget)
+template <typename R> constexpr int kList = sizeof(R) == 8 ? 16 : 24;
 
 template <typename R> struct alignas(4 * sizeof(R)) Q4 {
     R x, y, z, w;
@@ -307,7 +309,7 @@ template <int TI, typename R, bool MOLSITE, bool MULTI> struct Tile {
     // first-site (head) indices of a tile of charges (partners tested per molecule)
     unsigned short h[C::MOL && TI != T_LJ ? C::CAP : 1];
     Q4<R> e[C::ORI ? C::CAP : 1];      // orientation (dipole / quadrupole axis)
-    unsigned short lst[kList][64];     // per-lane hit lists (tile indices)
+    unsigned short lst[kList<R>][64];  // per-lane hit lists (tile indices)
     int j0[27], pre[28];
     double shift[27][3];
 };

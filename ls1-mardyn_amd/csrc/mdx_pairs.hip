@@ -312,7 +312,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         int m0 = 0;
         for (; m0 + kUnroll <= full; m0 += kUnroll) {
             // drain when some lane's list could overflow in this round
-            if (__any(cnt > kList - kUnroll)) drain();
+            if (__any(cnt > kList<R> - kUnroll)) drain();
             R r2v[kUnroll];
             int sv[kUnroll];
 #pragma unroll
@@ -338,7 +338,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             }
         }
         for (; m0 < per; ++m0) {  // the last rows, bounds-tested
-            if (__any(cnt > kList - 1)) drain();
+            if (__any(cnt > kList<R> - 1)) drain();
             const int c = ph + m0 * P;
             const int sidx = cand(min(c, nh - 1));
             const Q4<R> q = C::MOL ? T.m[sidx] : T.p[sidx];
