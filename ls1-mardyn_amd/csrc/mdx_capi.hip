@@ -26,6 +26,9 @@ __global__ void k_bin_scatter(int, const int *, const int *, const int *, int *)
 __global__ void k_bin_gather(int, MolArrays, MolArrays, const int *, const int *, const int *,
                              int *, const int *, int *, const Component *, int *, int *, int *,
                              int *);
+__global__ void k_bin_gather_point(int, MolArrays, MolArrays, const int *, const int *,
+                                   const int *, int *, const int *, int *, const Component *, Reps,
+                                   float *, float *, float *, int *);
 __global__ void k_scan_block(const int *, int *, int, int *);
 __global__ void k_scan_add(int *, int, const int *);
 __global__ void k_scan_total(const int *, const int *, int, int *);
@@ -62,6 +65,10 @@ struct mdx_context {
     std::vector<double> site_eps, site_sig, site_shift6, site_rcs, mix;
     bool has_mix = false;
     bool lj_at_origin = false;  // every LJ site sits at its molecule's centre
+    // every component is one LJ centre at its centre and nothing else: site i
+    // is molecule i, the gather builds the site cache, the pair kernel writes
+    // the molecule force and virial (no calcFM pass)
+    bool point_lj = false;
     std::vector<double> eps24, sig2, shift6;
     std::vector<double> shift_rc;  // radius of the shift tied to (eps, sigma); 0 = none
     double *d_eps24 = nullptr, *d_sig2 = nullptr, *d_shift6 = nullptr, *d_shift_rc = nullptr;
@@ -95,6 +102,7 @@ struct mdx_context {
     double *dl_tmp = nullptr;   // download staging (4 n)
     double *adj_site = nullptr, *adj_cot = nullptr, *adj_part = nullptr, *adj_out = nullptr;
     int *nsite_mol[kTypes] = {nullptr, nullptr, nullptr, nullptr};
+    int *ident = nullptr;  // 0 .. n (point_lj: molecule index of each site, site starts)
     SiteArrays sites[kTypes]{};
     std::vector<int *> scan_tmp;
     double *macro_part = nullptr, *lf_part = nullptr, *red = nullptr, *step_out = nullptr;
@@ -247,6 +255,11 @@ int parse_model(mdx_context *c, int ncomp, const double *desc, int len) {
         for (int s = 0; s < m.comp[ci].n[0]; ++s)
             for (int d = 0; d < 3; ++d)
                 if (m.comp[ci].body[0][s][d] != 0.0) c->lj_at_origin = false;
+    c->point_lj = c->lj_at_origin;
+    for (int ci = 0; ci < ncomp; ++ci) {
+        const Component &cp = m.comp[ci];
+        if (cp.n[0] != 1 || cp.n[1] != 0 || cp.n[2] != 0 || cp.n[3] != 0) c->point_lj = false;
+    }
     return MDX_OK;
 }
 
@@ -415,6 +428,12 @@ int alloc_mol(mdx_context *c, MolArrays &m, int n) {
 
 int alloc_sites(mdx_context *c, SiteArrays &s, int cap, int ncell, int n, int type) {
     memset(&s, 0, sizeof(s));
+    if (c->point_lj) {  // the rest are views of molecule arrays (site_view)
+        if (type != T_LJ || cap == 0) return MDX_OK;
+        float **fp[] = {&s.fx, &s.fy, &s.fz};
+        for (float **p : fp) TRY(dalloc(c, p, cap, true));
+        return dalloc(c, &s.ljid, cap, true);
+    }
     TRY(dalloc(c, &s.cstart, ncell + 1, true));
     TRY(dalloc(c, &s.mstart, n + 1, true));
     if (cap == 0) return MDX_OK;
@@ -500,6 +519,32 @@ void launch_reduce(mdx_context *c, const double *in, int nb, int width, double *
                        g1, out);
 }
 
+// The site arrays of type t as the kernels see them. For point_lj systems
+// the FP64 site and molecule positions are the current molecule positions,
+// the site outputs are the molecule force and virial rows, site i belongs to
+// molecule i and the per-cell site starts are the molecule starts.
+SiteArrays site_view(const mdx_context *c, int t) {
+    SiteArrays s = c->sites[t];
+    if (!c->point_lj || t != T_LJ) return s;
+    const MolArrays &m = c->mol[c->cur];
+    const size_t n = c->n;
+    s.x = s.mx = m.rx;
+    s.y = s.my = m.ry;
+    s.z = s.mz = m.rz;
+    s.fmx = s.fx;
+    s.fmy = s.fy;
+    s.fmz = s.fz;
+    s.mol = s.mstart = c->ident;
+    s.cstart = c->start;
+    s.Fx = c->F;
+    s.Fy = c->F + n;
+    s.Fz = c->F + 2 * n;
+    s.Vx = c->Vi;
+    s.Vy = c->Vi + n;
+    s.Vz = c->Vi + 2 * n;
+    return s;
+}
+
 // Re-bin (LinkedCells::update + exchangeMolecules' periodic wrap, which
 // leapfrog_pre already applied) and SoA cache build (setupSoACache).
 int launch_update(mdx_context *c) {
@@ -512,6 +557,15 @@ int launch_update(mdx_context *c) {
     TRY(scan_exclusive(c, c->count, c->start, c->ncell));
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, c->cell,
                        c->rank, c->start, c->perm);
+    if (c->point_lj) {
+        const SiteArrays &s = c->sites[T_LJ];
+        hipLaunchKernelGGL(k_bin_gather_point, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, src,
+                           dst, c->cell, c->perm, c->start, c->mcell, c->ls1id, c->mls1id,
+                           c->d_comps, reps(c), s.fx, s.fy, s.fz, s.ljid);
+        c->cur = 1 - c->cur;
+        HIPCHK(c, hipGetLastError());
+        return MDX_OK;
+    }
     hipLaunchKernelGGL(k_bin_gather, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, src, dst,
                        c->cell, c->perm, c->start, c->mcell, c->ls1id, c->mls1id, c->d_comps,
                        c->nsite_mol[0],
@@ -538,9 +592,10 @@ int launch_traverse(mdx_context *c) {
     a.rclj2 = c->g.rclj2;
     a.epsRFInvrc3 = c->g.epsRFInvrc3;
     for (int t = 0; t < kTypes; ++t) {
-        a.s[t] = c->sites[t];
+        a.s[t] = site_view(c, t);
         a.present[t] = c->nsites[t] > 0;
     }
+    a.vscale = c->point_lj ? 0.5 : 1.0;
     a.pp.eps24 = c->d_eps24;
     a.pp.sig2 = c->d_sig2;
     a.pp.shift6 = c->d_shift6;
@@ -574,9 +629,11 @@ int launch_traverse(mdx_context *c) {
     else
         hipLaunchKernelGGL(k_seg_reduce, dim3(R), dim3(256), 0, c->stream, c->macro_part, 4,
                            c->d_cell_off, c->red);
-    hipLaunchKernelGGL(k_calc_fm, dim3(nblk(c->n)), dim3(kBlock), 0, c->stream, c->n,
-                       c->mol[c->cur], c->d_comps, c->sites[0], c->sites[1], c->sites[2],
-                       c->sites[3], c->F, c->M, c->Vi);
+    // point_lj: the pair kernel wrote F and Vi; M stays 0 (no site offsets)
+    if (!c->point_lj)
+        hipLaunchKernelGGL(k_calc_fm, dim3(nblk(c->n)), dim3(kBlock), 0, c->stream, c->n,
+                           c->mol[c->cur], c->d_comps, c->sites[0], c->sites[1], c->sites[2],
+                           c->sites[3], c->F, c->M, c->Vi);
     hipLaunchKernelGGL(k_finalize_macro, dim3((R + 63) / 64), dim3(64), 0, c->stream, c->red,
                        c->red + 4 * R, c->step_out, c->step_idx, R);
     HIPCHK(c, hipGetLastError());
@@ -668,8 +725,15 @@ int state_setup(mdx_context *c, int64_t n64, const int32_t *cid_host) {
     TRY(dalloc(c, &c->count, c->ncell + 1, true));
     TRY(dalloc(c, &c->start, c->ncell + 1, true));
     for (int t = 0; t < kTypes; ++t) {
-        TRY(dalloc(c, &c->nsite_mol[t], n + 1, true));
+        c->nsite_mol[t] = nullptr;
+        if (!c->point_lj) TRY(dalloc(c, &c->nsite_mol[t], n + 1, true));
         TRY(alloc_sites(c, c->sites[t], c->nsites[t], c->ncell, n, t));
+    }
+    if (c->point_lj) {
+        std::vector<int> id((size_t)n + 1);
+        for (int i = 0; i <= n; ++i) id[i] = i;
+        TRY(dalloc(c, &c->ident, (size_t)n + 1, true));
+        HIPCHK(c, hipMemcpy(c->ident, id.data(), 4 * id.size(), hipMemcpyHostToDevice));
     }
     TRY(alloc_scan(c, std::max(n, c->ncell) + 1));
     TRY(dalloc(c, &c->macro_part, (size_t)c->ncell * 4, true));
@@ -1305,7 +1369,7 @@ int mdx_adjoint(mdx_context *c, int64_t n, const double *cot, double *dU_deps, d
         AdjArgs a;
         memset(&a, 0, sizeof(a));
         a.g = c->g;
-        a.s = c->sites[t];
+        a.s = site_view(c, t);
         a.pp.eps24 = c->d_eps24;
         a.pp.sig2 = c->d_sig2;
         a.pp.shift6 = c->d_shift6;
@@ -1339,7 +1403,7 @@ int mdx_adjoint(mdx_context *c, int64_t n, const double *cot, double *dU_deps, d
     if (vjp) {
         const size_t s0 = nsl, s1 = nsc;
         hipLaunchKernelGGL(adjoint_mol_kernel(), dim3(nblk(nn)), dim3(kBlock), 0, c->stream, nn,
-                           c->sites[T_LJ].mstart, c->sites[T_C].mstart,
+                           site_view(c, T_LJ).mstart, c->sites[T_C].mstart,
                            nsl ? g_site[0] : nullptr, g_site[0] + s0, g_site[0] + 2 * s0,
                            nsc ? g_site[1] : nullptr, g_site[1] + s1, g_site[1] + 2 * s1,
                            c->dl_tmp);

@@ -114,6 +114,8 @@ struct PairArgs {
     int present[kTypes];
     double *macro_part; // [ncell][4] per-cell sums {u6 (LJ), uXpoles, virial, rf}
     int accumulate;     // add to macro_part rows (a later i-site type) instead of writing
+    double vscale;      // factor on the written site virial: 1, or 0.5 where the site
+                        // arrays are the molecule arrays (calcFM's Vi = 0.5 sum V_site)
     unsigned long long *counters;  // [8] or null
 };
 

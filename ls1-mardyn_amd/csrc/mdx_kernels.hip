@@ -266,8 +266,11 @@ __global__ void k_fill_double(double *a, int n, double v) {
 }
 
 // Cell of every molecule (ls1 index -> inner index) and its arrival rank in
-// the cell. Lanes of a wavefront that share a cell (molecules are stored in
-// cell order, so a wave touches only a few cells) take one atomic per cell.
+// the cell. Molecules are stored in cell order, so a wave's lanes form a few
+// runs of equal cells: the first lane of each run takes one atomic for the
+// whole run, and all runs' atomics go out in the same instruction (one
+// round trip per wave; a cell that shows up in two runs gets two disjoint
+// rank ranges).
 __global__ __launch_bounds__(256) void k_bin_count(MolArrays s, Reps R, int n, int *cell,
                                                    int *rank, int *count, int *ls1id) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -284,17 +287,23 @@ __global__ __launch_bounds__(256) void k_bin_count(MolArrays s, Reps R, int n, i
         cell[i] = c;
         ls1id[i] = (ci[2] * g.cpd[1] + ci[1]) * g.cpd[0] + ci[0];
     }
-    unsigned long long active = __ballot(in);
-    while (active) {
-        const int leader = __ffsll((long long)active) - 1;
-        const int cl = __shfl(c, leader, 64);
-        const unsigned long long m = __ballot(c == cl) & active;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&count[cl], __popcll(m));
-        base = __shfl(base, leader, 64);
-        if (in && c == cl) rank[i] = base + __popcll(m & ((1ull << lane) - 1ull));
-        active &= ~m;
+    const int cprev = __shfl_up(c, 1, 64);
+    const bool head = in && (lane == 0 || cprev != c);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long valid = __ballot(in);
+    // my run's head: the highest head at or below my lane
+    const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane (all for 63)
+    const unsigned long long below = heads & upto;
+    const int h = in ? 63 - __clzll((long long)below) : 0;
+    int base = 0;
+    if (head) {
+        // the run ends at the next head or at the last valid lane
+        const unsigned long long after = heads & ~upto;
+        const int end = after ? __ffsll((long long)after) - 1 : 64 - __clzll((long long)valid);
+        base = atomicAdd(&count[c], end - lane);
     }
+    base = __shfl(base, h, 64);
+    if (in) rank[i] = base + (lane - h);
 }
 
 __global__ __launch_bounds__(256) void k_bin_scatter(int n, const int *__restrict__ cell,
@@ -307,7 +316,32 @@ __global__ __launch_bounds__(256) void k_bin_scatter(int n, const int *__restric
 
 // Stable order inside each cell (by previous position, which makes the sort
 // deterministic and keeps molecules that stay in their cell in order), then
-// the permutation of the state into cell order.
+// the permutation of the state into cell order; q is left to the caller.
+// Returns the destination index; *src_i, *cell_of the source and the cell.
+__device__ __forceinline__ int gather_common(int p, const MolArrays &src, const MolArrays &dst,
+                                             const int *__restrict__ cell,
+                                             const int *__restrict__ perm,
+                                             const int *__restrict__ start, int *newcell,
+                                             const int *__restrict__ ls1id, int *newls1id,
+                                             int *src_i, int *cell_of) {
+    int i = perm[p];
+    int c = cell[i];
+    int b = start[c], e = start[c + 1];
+    int r = 0;
+    for (int k = b; k < e; ++k) r += (perm[k] < i);
+    int d = b + r;
+    dst.rx[d] = src.rx[i]; dst.ry[d] = src.ry[i]; dst.rz[d] = src.rz[i];
+    dst.vx[d] = src.vx[i]; dst.vy[d] = src.vy[i]; dst.vz[d] = src.vz[i];
+    dst.Lx[d] = src.Lx[i]; dst.Ly[d] = src.Ly[i]; dst.Lz[d] = src.Lz[i];
+    dst.cid[d] = src.cid[i];
+    dst.orig[d] = src.orig[i];
+    newcell[d] = c;
+    newls1id[d] = ls1id[i];
+    *src_i = i;
+    *cell_of = c;
+    return d;
+}
+
 __global__ __launch_bounds__(256) void k_bin_gather(int n, MolArrays src, MolArrays dst,
                                                     const int *__restrict__ cell,
                                                     const int *__restrict__ perm,
@@ -318,26 +352,43 @@ __global__ __launch_bounds__(256) void k_bin_gather(int n, MolArrays src, MolArr
                                                     int *nsite3) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    int i = perm[p];
-    int c = cell[i];
-    int b = start[c], e = start[c + 1];
-    int r = 0;
-    for (int k = b; k < e; ++k) r += (perm[k] < i);
-    int d = b + r;
-    dst.rx[d] = src.rx[i]; dst.ry[d] = src.ry[i]; dst.rz[d] = src.rz[i];
-    dst.vx[d] = src.vx[i]; dst.vy[d] = src.vy[i]; dst.vz[d] = src.vz[i];
+    int i, c;
+    const int d = gather_common(p, src, dst, cell, perm, start, newcell, ls1id, newls1id, &i, &c);
     dst.qw[d] = src.qw[i]; dst.qx[d] = src.qx[i]; dst.qy[d] = src.qy[i]; dst.qz[d] = src.qz[i];
-    dst.Lx[d] = src.Lx[i]; dst.Ly[d] = src.Ly[i]; dst.Lz[d] = src.Lz[i];
-    int cid = src.cid[i];
-    dst.cid[d] = cid;
-    dst.orig[d] = src.orig[i];
-    newcell[d] = c;
-    newls1id[d] = ls1id[i];
-    const Component &cp = comps[cid];
+    const Component &cp = comps[src.cid[i]];
     nsite0[d] = cp.n[0];
     nsite1[d] = cp.n[1];
     nsite2[d] = cp.n[2];
     nsite3[d] = cp.n[3];
+}
+
+// The same for systems whose every component is one LJ centre at the
+// molecule's centre and nothing else: site j is molecule j, so the gather
+// also does the SoA cache build of k_build_sites for them: q normalised
+// (FullMolecule.cpp:719), the FP32 position relative to the cell corner and
+// the LJ lookup id. Site FP64 positions, forces and virials are the molecule
+// arrays themselves.
+__global__ __launch_bounds__(256) void k_bin_gather_point(
+    int n, MolArrays src, MolArrays dst, const int *__restrict__ cell,
+    const int *__restrict__ perm, const int *__restrict__ start, int *newcell,
+    const int *__restrict__ ls1id, int *newls1id, const Component *__restrict__ comps, Reps R,
+    float *fx, float *fy, float *fz, int *ljid) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int i, c;
+    const int d = gather_common(p, src, dst, cell, perm, start, newcell, ls1id, newls1id, &i, &c);
+    double q[4] = {src.qw[i], src.qx[i], src.qy[i], src.qz[i]};
+    double qs = 1. / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int k = 0; k < 4; ++k) q[k] *= qs;
+    dst.qw[d] = q[0]; dst.qx[d] = q[1]; dst.qy[d] = q[2]; dst.qz[d] = q[3];
+    const int rep = cell_replica(R, c);
+    const Geom &g = R.geom[rep];
+    const int local = c - R.cell_off[rep];
+    const int cx = local % g.n[0], cy = (local / g.n[0]) % g.n[1], cz = local / (g.n[0] * g.n[1]);
+    fx[d] = (float)(src.rx[i] - (g.bbmin[0] + cx * g.cl[0]));
+    fy[d] = (float)(src.ry[i] - (g.bbmin[1] + cy * g.cl[1]));
+    fz[d] = (float)(src.rz[i] - (g.bbmin[2] + cz * g.cl[2]));
+    ljid[d] = comps[src.cid[i]].lj_base;
 }
 
 // ------------------------------------------------------------- scan (int)

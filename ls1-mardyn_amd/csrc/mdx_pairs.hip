@@ -436,7 +436,9 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
                 }
             if (valid && ph == 0) {
                 S.Fx[i] = acc.f[0]; S.Fy[i] = acc.f[1]; S.Fz[i] = acc.f[2];
-                S.Vx[i] = acc.v[0]; S.Vy[i] = acc.v[1]; S.Vz[i] = acc.v[2];
+                S.Vx[i] = a.vscale * acc.v[0];
+                S.Vy[i] = a.vscale * acc.v[1];
+                S.Vz[i] = a.vscale * acc.v[2];
                 if constexpr (TI >= T_D) {
                     S.Mx[i] = acc.m[0]; S.My[i] = acc.m[1]; S.Mz[i] = acc.m[2];
                 }
