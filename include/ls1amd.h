@@ -117,6 +117,31 @@ int mdx_set_pair_config(mdx_context *ctx, int cells_per_block, int blocks_per_xc
  * (may be NULL) = {cells_per_block, blocks_per_xcd}; best_ms = its time. */
 int mdx_autotune(mdx_context *ctx, int reps, int32_t *chosen, double *best_ms);
 
+/* The NVT tail of Simulation::simulateOneTimestep (Simulation.cpp:1161-1266)
+ * inside every mdx_step, per replica (nrep = the context's replica count):
+ *  - lrc != 0: the homogeneous long-range correction (Homogeneous::init /
+ *    calculateLongRange, src/longRange/Homogeneous.cpp:21-146): VirialCorr /
+ *    (3N) is added to every molecule's virial diagonal before calcFM's site
+ *    sums, and UpotCorr / VirialCorr to the step record's Upot and virial
+ *    (Domain::calculateGlobalValues, Domain.cpp:187-188). Only unshifted LJ
+ *    table entries contribute, as in the reference. The default (0) matches
+ *    <longrange type="none">.
+ *  - T_target[r] > 0: after upd_postF, the global thermostat's temperature
+ *    and scaling factors beta_trans = (3 N Ti / sum m v^2)^0.4, beta_rot =
+ *    (rotDOF Ti / sum I w^2)^0.4 (Domain.cpp:219-259, Tfactor 1), the
+ *    explosion heuristic when a beta is below 0.9 and explosion_heuristics
+ *    != 0 (Domain.cpp:261-313), and VelocityScalingThermostat::apply
+ *    (VelocityScalingThermostat.cpp:70-90): v *= beta_trans, D *= beta_rot.
+ *    T_target NULL or all <= 0: no thermostat (the NVE path, the default).
+ * Settings are reset by mdx_set_domain / mdx_set_replicas. */
+int mdx_set_thermostat(mdx_context *ctx, int nrep, const double *T_target, int lrc,
+                       int explosion_heuristics);
+
+/* Per-replica record of the last step's NVT tail, out[nrep][8] = {T_target,
+ * UpotCorr, VirialCorr, VirialCorr/(3N), T, beta_trans, beta_rot, explosion
+ * (0/1)}; T is computed before the velocities are scaled (Domain.cpp:226-230). */
+int mdx_thermostat_state(mdx_context *ctx, double *out);
+
 /* MDX_PREC_FP64 (default) or MDX_PREC_FP32_FP64ACC. */
 int mdx_set_precision(mdx_context *ctx, int precision);
 

@@ -36,10 +36,13 @@ __global__ void k_cell_site_start(int, const int *, const int *, int *);
 __global__ void k_build_sites(int, MolArrays, Reps, const int *, const Component *, SiteArrays,
                               SiteArrays, SiteArrays, SiteArrays);
 __global__ void k_calc_fm(int, MolArrays, const Component *, SiteArrays, SiteArrays, SiteArrays,
-                          SiteArrays, double *, double *, double *);
+                          SiteArrays, double *, double *, double *, Reps, const double *);
+__global__ void k_velocity_scale(MolArrays, double *, double *, const Component *, Reps,
+                                 const double *, int);
 __global__ void k_scatter_orig(int, const int *, const double *, double *, int, int);
-__global__ void k_finalize_macro(const double *, double *, double *, const int *, int);
-__global__ void k_finalize_lf(const double *, double *, const int *, int);
+__global__ void k_finalize_macro(const double *, double *, double *, const int *, int,
+                                 const double *);
+__global__ void k_finalize_lf(const double *, double *, const int *, int, double *, int);
 __global__ void k_step_advance(int *);
 __global__ void k_enum_pairs(Geom, int, int *);
 }  // namespace mdx
@@ -69,6 +72,13 @@ struct mdx_context {
     // is molecule i, the gather builds the site cache, the pair kernel writes
     // the molecule force and virial (no calcFM pass)
     bool point_lj = false;
+    // NVT tail of the step (Domain::calculateGlobalValues, Homogeneous LRC,
+    // velocity-scaling thermostat), off by default (the NVE path)
+    bool lrc = false, thermostat = false;
+    int heuristics = 1;
+    std::vector<double> T_target;    // [nrep]; <= 0: no thermostat for that replica
+    std::vector<double> comp_count;  // [nrep][ncomp] molecules per component
+    double *d_thermo = nullptr;      // [nrep][kThermoW]
     std::vector<double> eps24, sig2, shift6;
     std::vector<double> shift_rc;  // radius of the shift tied to (eps, sigma); 0 = none
     double *d_eps24 = nullptr, *d_sig2 = nullptr, *d_shift6 = nullptr, *d_shift_rc = nullptr;
@@ -387,12 +397,134 @@ void set_cutoff_terms(mdx_context *c) {
     }
 }
 
+// Homogeneous::init + calculateLongRange (src/longRange/Homogeneous.cpp:21-146)
+// for count[i] molecules of component i in a box of volume `volume`:
+// out = {UpotCorr, VirialCorr, VirialCorr / (3N)}. Only LJ table entries with
+// shift6 == 0 contribute, as the reference reads uLJshift6 from Comp2Param;
+// MySelbstTerm is the reaction-field self term of each molecule's dipole
+// (charges' first moment plus point dipoles). The radial integrals are the
+// reference's _TICC/_TICS/_TISS (Homogeneous.cpp:158-196).
+namespace lrc {
+double ticc_u(int n, double rc, double s2) { return -pow(rc, 2 * n + 3) / (pow(s2, n) * (2 * n + 3)); }
+double tics_u(int n, double rc, double s2, double t) {
+    return -(pow(rc + t, 2 * n + 3) - pow(rc - t, 2 * n + 3)) * rc /
+               (4 * pow(s2, n) * t * (n + 1) * (2 * n + 3)) +
+           (pow(rc + t, 2 * n + 4) - pow(rc - t, 2 * n + 4)) /
+               (4 * pow(s2, n) * t * (n + 1) * (2 * n + 3) * (2 * n + 4));
+}
+double tiss_u(int n, double rc, double s2, double t1, double t2) {
+    const double tp = t1 + t2, tm = t1 - t2;
+    return -(pow(rc + tp, 2 * n + 4) - pow(rc + tm, 2 * n + 4) - pow(rc - tm, 2 * n + 4) +
+             pow(rc - tp, 2 * n + 4)) *
+               rc / (8 * pow(s2, n) * t1 * t2 * (n + 1) * (2 * n + 3) * (2 * n + 4)) +
+           (pow(rc + tp, 2 * n + 5) - pow(rc + tm, 2 * n + 5) - pow(rc - tm, 2 * n + 5) +
+            pow(rc - tp, 2 * n + 5)) /
+               (8 * pow(s2, n) * t1 * t2 * (n + 1) * (2 * n + 3) * (2 * n + 4) * (2 * n + 5));
+}
+double ticc_v(int n, double rc, double s2) { return 2 * n * ticc_u(n, rc, s2); }
+double tics_v(int n, double rc, double s2, double t) {
+    return -(pow(rc + t, 2 * n + 2) - pow(rc - t, 2 * n + 2)) * rc * rc /
+               (4 * pow(s2, n) * t * (n + 1)) -
+           3 * tics_u(n, rc, s2, t);
+}
+double tiss_v(int n, double rc, double s2, double t1, double t2) {
+    const double tp = t1 + t2, tm = t1 - t2;
+    return -(pow(rc + tp, 2 * n + 3) - pow(rc + tm, 2 * n + 3) - pow(rc - tm, 2 * n + 3) +
+             pow(rc - tp, 2 * n + 3)) *
+               rc * rc / (8 * pow(s2, n) * t1 * t2 * (n + 1) * (2 * n + 3)) -
+           3 * tiss_u(n, rc, s2, t1, t2);
+}
+}  // namespace lrc
+
+int lrc_terms(mdx_context *c, const double *count, double volume, double *out) {
+    const Model &m = c->model;
+    const int nl = m.nlj;
+    double upot = 0., vir = 0., self = 0., N = 0.;
+    for (int i = 0; i < m.ncomp; ++i) {
+        const Component &ci = m.comp[i];
+        N += count[i];
+        double cb[3] = {0, 0, 0};
+        for (int s = 0; s < ci.n[T_C]; ++s)
+            for (int d = 0; d < 3; ++d) cb[d] += ci.param[T_C][s] * ci.body[T_C][s][d];
+        for (int s = 0; s < ci.n[T_D]; ++s) {
+            double ev = 0.;
+            for (int d = 0; d < 3; ++d) ev += ci.e[T_D][s][d] * ci.e[T_D][s][d];
+            const double norm = 1.0 / sqrt(ev);
+            for (int d = 0; d < 3; ++d) cb[d] += ci.param[T_D][s] * ci.e[T_D][s][d] * norm;
+        }
+        double my2 = 0.;
+        for (int d = 0; d < 3; ++d) my2 += cb[d] * cb[d];
+        self += my2 * count[i];
+        for (int j = 0; j < m.ncomp; ++j) {
+            const Component &cj = m.comp[j];
+            for (int si = 0; si < ci.n[T_LJ]; ++si) {
+                const double *a = ci.body[T_LJ][si];
+                const double t1 = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+                for (int sj = 0; sj < cj.n[T_LJ]; ++sj) {
+                    const double *b = cj.body[T_LJ][sj];
+                    const double t2 = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+                    if (t1 + t2 >= c->rc_lj)
+                        return fail(c, MDX_ERR_ARG, "long-range correction: rcLJ too small");
+                    const size_t k = (size_t)(ci.lj_base + si) * nl + cj.lj_base + sj;
+                    if (c->shift6[k] != 0.0) continue;
+                    const double fac = count[i] * count[j] * c->eps24[k], s2 = c->sig2[k];
+                    const double rc = c->rc_lj;
+                    if (t1 == 0. && t2 == 0.) {
+                        upot += fac * (lrc::ticc_u(-6, rc, s2) - lrc::ticc_u(-3, rc, s2));
+                        vir += fac * (lrc::ticc_v(-6, rc, s2) - lrc::ticc_v(-3, rc, s2));
+                    } else if (t1 != 0. && t2 != 0.) {
+                        upot += fac * (lrc::tiss_u(-6, rc, s2, t1, t2) - lrc::tiss_u(-3, rc, s2, t1, t2));
+                        vir += fac * (lrc::tiss_v(-6, rc, s2, t1, t2) - lrc::tiss_v(-3, rc, s2, t1, t2));
+                    } else {
+                        const double t = t2 == 0. ? t1 : t2;
+                        upot += fac * (lrc::tics_u(-6, rc, s2, t) - lrc::tics_u(-3, rc, s2, t));
+                        vir += fac * (lrc::tics_v(-6, rc, s2, t) - lrc::tics_v(-3, rc, s2, t));
+                    }
+                }
+            }
+        }
+    }
+    if (N <= 0) {
+        out[0] = out[1] = out[2] = 0.;
+        return MDX_OK;
+    }
+    const double rho = N / volume;
+    const double fac = M_PI * rho / (3. * N);
+    const double epsRFInvrc3 =
+        2. * (c->eps_rf - 1.) / ((c->rc * c->rc * c->rc) * (2. * c->eps_rf + 1.));
+    const double myself = -0.5 * epsRFInvrc3 * self;
+    out[0] = fac * upot + myself;
+    out[1] = -fac * vir + 3. * myself;
+    out[2] = out[1] / (3. * N);
+    return MDX_OK;
+}
+
+// Per-replica NVT record on the device: target T and the long-range terms;
+// the step outputs start at T 0, betas 1, no explosion.
+int upload_thermo(mdx_context *c) {
+    const int R = c->nrep, nc = c->model.ncomp;
+    std::vector<double> th((size_t)kThermoW * R, 0.0);
+    for (int r = 0; r < R; ++r) {
+        double *t = th.data() + (size_t)kThermoW * r;
+        t[kThTarget] = r < (int)c->T_target.size() ? c->T_target[r] : 0.0;
+        t[kThBetaT] = t[kThBetaR] = 1.0;
+        if (c->lrc && (int)c->comp_count.size() == R * nc) {  // counts of the uploaded state
+            const Geom &g = c->geoms[r];
+            TRY(lrc_terms(c, c->comp_count.data() + (size_t)nc * r, g.L[0] * g.L[1] * g.L[2],
+                          t + kThUcorr));
+        }
+    }
+    HIPCHK(c, hipMemcpy(c->d_thermo, th.data(), 8 * th.size(), hipMemcpyHostToDevice));
+    return MDX_OK;
+}
+
 // Replica geometries on the device; reduction buffers sized for nrep.
 int upload_reps(mdx_context *c) {
     const int R = c->nrep;
     TRY(dalloc(c, &c->d_geoms, R));
     TRY(dalloc(c, &c->d_cell_off, R + 1));
     TRY(dalloc(c, &c->red, 12 * (size_t)R));
+    TRY(dalloc(c, &c->d_thermo, (size_t)kThermoW * R));
     HIPCHK(c, hipMemcpy(c->d_geoms, c->geoms.data(), sizeof(Geom) * R, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_cell_off, c->cell_off.data(), 4 * (size_t)(R + 1),
                         hipMemcpyHostToDevice));
@@ -596,6 +728,7 @@ int launch_traverse(mdx_context *c) {
         a.present[t] = c->nsites[t] > 0;
     }
     a.vscale = c->point_lj ? 0.5 : 1.0;
+    a.thermo = (c->lrc && c->point_lj) ? c->d_thermo : nullptr;
     a.pp.eps24 = c->d_eps24;
     a.pp.sig2 = c->d_sig2;
     a.pp.shift6 = c->d_shift6;
@@ -633,9 +766,11 @@ int launch_traverse(mdx_context *c) {
     if (!c->point_lj)
         hipLaunchKernelGGL(k_calc_fm, dim3(nblk(c->n)), dim3(kBlock), 0, c->stream, c->n,
                            c->mol[c->cur], c->d_comps, c->sites[0], c->sites[1], c->sites[2],
-                           c->sites[3], c->F, c->M, c->Vi);
+                           c->sites[3], c->F, c->M, c->Vi, reps(c),
+                           c->lrc ? c->d_thermo : nullptr);
     hipLaunchKernelGGL(k_finalize_macro, dim3((R + 63) / 64), dim3(64), 0, c->stream, c->red,
-                       c->red + 4 * R, c->step_out, c->step_idx, R);
+                       c->red + 4 * R, c->step_out, c->step_idx, R,
+                       c->lrc ? c->d_thermo : nullptr);
     HIPCHK(c, hipGetLastError());
     return MDX_OK;
 }
@@ -657,7 +792,11 @@ int launch_lf_post(mdx_context *c, double dt) {
         hipLaunchKernelGGL(k_seg_reduce, dim3(R), dim3(256), 0, c->stream, c->lf_part, 4,
                            c->d_lf_off, c->red + 8 * R);
     hipLaunchKernelGGL(k_finalize_lf, dim3((R + 63) / 64), dim3(64), 0, c->stream, c->red + 8 * R,
-                       c->step_out, c->step_idx, R);
+                       c->step_out, c->step_idx, R, c->thermostat ? c->d_thermo : nullptr,
+                       c->heuristics);
+    if (c->thermostat)
+        hipLaunchKernelGGL(k_velocity_scale, dim3(nblk(c->n)), dim3(kBlock), 0, c->stream,
+                           c->mol[c->cur], c->F, c->M, c->d_comps, reps(c), c->d_thermo, c->n);
     HIPCHK(c, hipGetLastError());
     return MDX_OK;
 }
@@ -780,7 +919,18 @@ int state_setup(mdx_context *c, int64_t n64, const int32_t *cid_host) {
     HIPCHK(c, hipMemsetAsync(c->M, 0, 3 * (size_t)n * 8, c->stream));
     HIPCHK(c, hipMemsetAsync(c->Vi, 0, 3 * (size_t)n * 8, c->stream));
     c->cur = 0;
-    return MDX_OK;
+    // molecules per component and replica (the long-range correction's N_i)
+    {
+        const int nc = c->model.ncomp;
+        c->comp_count.assign((size_t)c->nrep * nc, 0.0);
+        int64_t m0 = 0;
+        for (int r = 0; r < c->nrep; ++r) {
+            const int64_t cnt = c->nrep > 1 ? c->rep_n[r] : n64;
+            for (int64_t k = 0; k < cnt; ++k) c->comp_count[(size_t)r * nc + cid_host[m0 + k]] += 1.0;
+            m0 += cnt;
+        }
+    }
+    return upload_thermo(c);
 }
 
 }  // namespace
@@ -884,7 +1034,11 @@ int mdx_set_domain(mdx_context *c, const double *box, double rc, double rc_lj, i
     c->cap_steps = 0;
     TRY(ensure_steps(c, 1));
     set_cutoff_terms(c);
+    c->T_target.clear();  // thermostat settings belong to a domain / replica set
+    c->comp_count.clear();
+    c->lrc = c->thermostat = false;
     TRY(upload_reps(c));
+    TRY(upload_thermo(c));
     build_tables(c);
     TRY(upload_tables(c));
     c->have_domain = true;
@@ -924,7 +1078,11 @@ int mdx_set_replicas(mdx_context *c, int nrep, const double *boxes, const int64_
     set_cutoff_terms(c);
     build_tables(c);
     TRY(upload_tables(c));
+    c->T_target.clear();  // thermostat settings belong to a domain / replica set
+    c->comp_count.clear();
+    c->lrc = c->thermostat = false;
     TRY(upload_reps(c));
+    TRY(upload_thermo(c));
     c->have_domain = true;
     return MDX_OK;
 }
@@ -935,6 +1093,35 @@ int mdx_set_pair_config(mdx_context *c, int cells_per_block, int blocks_per_xcd)
     c->cpb = cells_per_block;
     c->blocks_per_xcd = blocks_per_xcd;
     drop_graphs(c);
+    return MDX_OK;
+}
+
+int mdx_set_thermostat(mdx_context *c, int nrep, const double *T_target, int lrc,
+                       int explosion_heuristics) {
+    if (!c) return MDX_ERR_ARG;
+    TRY(need(c, true, true, false));
+    if (nrep != c->nrep) return fail(c, MDX_ERR_ARG, "nrep differs from the replica count");
+    HIPCHK(c, hipSetDevice(c->device));
+    c->T_target.assign((size_t)nrep, 0.0);
+    c->thermostat = false;
+    if (T_target)
+        for (int r = 0; r < nrep; ++r) {
+            c->T_target[r] = T_target[r];
+            if (T_target[r] > 0.0) c->thermostat = true;
+        }
+    c->lrc = lrc != 0;
+    c->heuristics = explosion_heuristics != 0;
+    drop_graphs(c);
+    return upload_thermo(c);
+}
+
+int mdx_thermostat_state(mdx_context *c, double *out) {
+    if (!c || !out) return MDX_ERR_ARG;
+    TRY(need(c, true, true, false));
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, c->d_thermo, sizeof(double) * kThermoW * c->nrep,
+                        hipMemcpyDeviceToHost));
     return MDX_OK;
 }
 

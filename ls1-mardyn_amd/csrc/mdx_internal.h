@@ -103,6 +103,14 @@ __device__ __forceinline__ int cell_replica(const Reps &R, int c) {
     return R.cell_rep ? R.cell_rep[c] : 0;
 }
 
+// Per-replica record of the NVT tail (global thermostat, homogeneous LRC):
+// target T, Upot / virial corrections, per-molecule virial addition, then the
+// step's T, beta_trans, beta_rot and explosion flag.
+enum {
+    kThTarget = 0, kThUcorr = 1, kThWcorr = 2, kThVi = 3,
+    kThT = 4, kThBetaT = 5, kThBetaR = 6, kThExplode = 7, kThermoW = 8
+};
+
 struct PairArgs {
     Reps R;
     double rc2, rclj2, epsRFInvrc3;   // shared by all replicas
@@ -116,6 +124,8 @@ struct PairArgs {
     int accumulate;     // add to macro_part rows (a later i-site type) instead of writing
     double vscale;      // factor on the written site virial: 1, or 0.5 where the site
                         // arrays are the molecule arrays (calcFM's Vi = 0.5 sum V_site)
+    const double *thermo;  // [nrep][kThermoW] when the written virial also takes the
+                           // long-range correction's per-molecule addition, else null
     unsigned long long *counters;  // [8] or null
 };
 

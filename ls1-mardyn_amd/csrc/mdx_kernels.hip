@@ -3,6 +3,8 @@
 // Step = leapfrog_pre (+ periodic wrap) -> bin_count -> scan -> bin_scatter
 //        -> bin_gather (stable counting sort) -> site scans -> build_sites
 //        -> pair kernels (one per i-site type) -> calc_fm -> leapfrog_post.
+// Systems of one-centre LJ molecules only: bin_gather_point (gather + site
+// cache) -> LJ pair kernel (writes molecule F, Vi) -> leapfrog_post.
 //
 // Pair kernels: one 64-lane wavefront per cell. Lane = (i-site, j-phase):
 // the cell's sites of the i type occupy Ipad = pow2ceil(n) lanes and the
@@ -509,7 +511,8 @@ __global__ __launch_bounds__(256) void k_build_sites(int n, MolArrays s, Reps R,
 __global__ __launch_bounds__(256) void k_calc_fm(int n, MolArrays s,
                                                  const Component *__restrict__ comps,
                                                  SiteArrays s0, SiteArrays s1, SiteArrays s2,
-                                                 SiteArrays s3, double *F, double *M, double *Vi) {
+                                                 SiteArrays s3, double *F, double *M, double *Vi,
+                                                 Reps R, const double *__restrict__ thermo) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Component &c = comps[s.cid[i]];
@@ -540,7 +543,11 @@ __global__ __launch_bounds__(256) void k_calc_fm(int n, MolArrays s,
     for (int d = 0; d < 3; ++d) {
         F[d * n + i] = f[d];
         M[d * n + i] = m[d] + tm[d];
-        Vi[d * n + i] = 0.5 * tv[d];
+        // Homogeneous::calculateLongRange adds VirialCorr / (3N) to every
+        // molecule before calcFM adds its sites (Homogeneous.cpp:139-145)
+        Vi[d * n + i] =
+            thermo ? __dadd_rn(__dmul_rn(0.5, tv[d]), thermo[kThermoW * mol_replica(R, i) + kThVi])
+                   : 0.5 * tv[d];
     }
 }
 
@@ -552,7 +559,8 @@ __global__ __launch_bounds__(256) void k_calc_fm(int n, MolArrays s,
 // One thread per replica; raw, macro are [nrep][4], the step record is
 // [step][nrep][6].
 __global__ void k_finalize_macro(const double *raw, double *macro, double *step_out,
-                                 const int *step_idx, int nrep) {
+                                 const int *step_idx, int nrep,
+                                 const double *__restrict__ thermo) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrep) return;
     raw += 4 * r;
@@ -565,17 +573,97 @@ __global__ void k_finalize_macro(const double *raw, double *macro, double *step_
     double *rec = step_out + 6 * ((size_t)step_idx[0] * nrep + r);
     rec[0] = u6 / 6.0 + ux + rf;
     rec[1] = vir + 3.0 * rf;
+    if (thermo) {  // Domain::calculateGlobalValues (Domain.cpp:187-188)
+        rec[0] = rec[0] + thermo[kThermoW * r + kThUcorr];
+        rec[1] = rec[1] + thermo[kThermoW * r + kThWcorr];
+    }
     rec[4] = u6 / 6.0;
     rec[5] = ux + rf;
 }
 
 __global__ void k_finalize_lf(const double *sums, double *step_out, const int *step_idx,
-                              int nrep) {
+                              int nrep, double *thermo, int heuristics) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrep) return;
     double *rec = step_out + 6 * ((size_t)step_idx[0] * nrep + r);
     rec[2] = sums[4 * r];
     rec[3] = sums[4 * r + 1];
+    if (!thermo) return;
+    // Domain::calculateGlobalValues, global thermostat, Tfactor 1
+    // (Domain.cpp:219-313): temperature and scaling factors; the explosion
+    // heuristic replaces them when one is below MIN_BETA = 0.9 (its error
+    // counter never rises above 0 in the reference, so it stays armed)
+    double *th = thermo + kThermoW * r;
+    const double Ti = th[kThTarget];
+    const double summv2 = sums[4 * r], N = sums[4 * r + 2], rot = sums[4 * r + 3];
+    const double sumIw2 = rot > 0 ? sums[4 * r + 1] : 0.0;
+    th[kThT] = N > 0 ? (summv2 + sumIw2) / (3 * N + rot) : Ti;
+    double bt = 1.0, br = 1.0;
+    if (Ti > 0.0 && N > 0) {
+        bt = pow(3.0 * N * Ti / summv2, 0.4);
+        br = sumIw2 == 0.0 ? 1.0 : pow(rot * Ti / sumIw2, 0.4);
+    }
+    double explode = 0.0;
+    if ((bt < 0.9 || br < 0.9) && heuristics) {
+        explode = 1.0;
+        bt = 1.0;
+        br = pow(br, 0.0091);
+    }
+    th[kThBetaT] = bt;
+    th[kThBetaR] = br;
+    th[kThExplode] = explode;
+}
+
+// The explosion capping (Domain.cpp:263-297: a molecule's translational
+// kinetic energy above 10 Ti, or rotational above 3 rotDOF Ti, is scaled down,
+// v with F and D with M) when the step's heuristic fired, then
+// VelocityScalingThermostat::apply for the global thermostat without a
+// directed velocity (VelocityScalingThermostat.cpp:70-90): v *= beta_trans,
+// D *= beta_rot, for every molecule of a replica with a target temperature.
+__global__ __launch_bounds__(256) void k_velocity_scale(MolArrays s, double *F, double *M,
+                                                        const Component *__restrict__ comps,
+                                                        Reps R, const double *__restrict__ thermo,
+                                                        int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *th = thermo + kThermoW * mol_replica(R, i);
+    const double Ti = th[kThTarget];
+    if (!(Ti > 0.0)) return;
+    const Component &c = comps[s.cid[i]];
+    double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
+    double L[3] = {s.Lx[i], s.Ly[i], s.Lz[i]};
+    if (th[kThExplode] != 0.0) {
+        const double limit = 10.0 * Ti;
+        const double ut = 0.5 * c.m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (ut > limit) {
+            const double vc = sqrt(limit / ut);
+            for (int d = 0; d < 3; ++d) {
+                v[d] *= vc;
+                F[d * n + i] *= vc;
+            }
+        }
+        if (c.rotdof > 0) {
+            const double lr = 3.0 * c.rotdof * Ti;
+            const double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
+            double w[3], Iw2 = 0.;
+            qrotinv(q, L, w);
+            for (int d = 0; d < 3; ++d) {
+                w[d] *= c.invI[d];
+                Iw2 += c.I[d] * w[d] * w[d];
+            }
+            const double ur = 0.5 * Iw2;
+            if (ur > lr) {
+                const double dc = sqrt(lr / ur);
+                for (int d = 0; d < 3; ++d) {
+                    L[d] *= dc;
+                    M[d * n + i] *= dc;
+                }
+            }
+        }
+    }
+    const double bt = th[kThBetaT], br = th[kThBetaR];
+    s.vx[i] = v[0] * bt; s.vy[i] = v[1] * bt; s.vz[i] = v[2] * bt;
+    s.Lx[i] = L[0] * br; s.Ly[i] = L[1] * br; s.Lz[i] = L[2] * br;
 }
 
 // Segmented fixed-order sum: block r adds rows [off[r], off[r+1]) of a

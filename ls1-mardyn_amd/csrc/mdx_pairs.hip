@@ -369,12 +369,14 @@ template <int TI, typename R, bool COUNT, bool MOLSITE, bool MULTI>
 __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE, MULTI> &T,
                                           int cell, int lane, unsigned long long &ntest) {
     double wmac[3] = {0, 0, 0};
+    bool empty;
     {
         const int rep = cell_replica(a.R, cell);
         const Geom &g = a.R.geom[rep];
         const int cbase = a.R.cell_off[rep], local = cell - cbase;
         const SiteArrays &S = a.s[TI];
         const int i_begin = S.cstart[cell], i_end = S.cstart[cell + 1];
+        empty = i_begin == i_end;
         const int cx = local % g.n[0], cy = (local / g.n[0]) % g.n[1],
                   cz = local / (g.n[0] * g.n[1]);
         R eps24 = R(0), sig2 = R(0), shift6 = R(0);
@@ -436,9 +438,15 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
                 }
             if (valid && ph == 0) {
                 S.Fx[i] = acc.f[0]; S.Fy[i] = acc.f[1]; S.Fz[i] = acc.f[2];
-                S.Vx[i] = a.vscale * acc.v[0];
-                S.Vy[i] = a.vscale * acc.v[1];
-                S.Vz[i] = a.vscale * acc.v[2];
+                double vv[3] = {__dmul_rn(a.vscale, acc.v[0]), __dmul_rn(a.vscale, acc.v[1]),
+                                __dmul_rn(a.vscale, acc.v[2])};
+                if (a.thermo) {  // Homogeneous.cpp:139-145, then calcFM's own sum
+                    const double w = a.thermo[kThermoW * cell_replica(a.R, cell) + kThVi];
+                    for (int k = 0; k < 3; ++k) vv[k] = __dadd_rn(vv[k], w);
+                }
+                S.Vx[i] = vv[0];
+                S.Vy[i] = vv[1];
+                S.Vz[i] = vv[2];
                 if constexpr (TI >= T_D) {
                     S.Mx[i] = acc.m[0]; S.My[i] = acc.m[1]; S.Mz[i] = acc.m[2];
                 }
@@ -446,6 +454,11 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             wmac[0] += acc.u;
             wmac[2] += acc.rf;
         }
+    }
+    if (empty) {  // zero row; nothing to add to an existing one
+        if (lane == 0 && !a.accumulate)
+            for (int k = 0; k < 4; ++k) a.macro_part[4 * (size_t)cell + k] = 0.0;
+        return;
     }
     for (int k = 0; k < 3; ++k)
         for (int o = 32; o > 0; o >>= 1) wmac[k] += shfl_xor_d(wmac[k], o);

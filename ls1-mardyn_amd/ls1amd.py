@@ -30,6 +30,7 @@ EXPORTS = (
     "mdx_cell_ids", "mdx_cell_geometry", "mdx_cell_pairs", "mdx_set_counting", "mdx_counters",
     "mdx_upload_device", "mdx_set_timing", "mdx_last_kernel_ms", "mdx_set_charges",
     "mdx_adjoint", "mdx_lj_tables", "mdx_set_replicas", "mdx_set_pair_config", "mdx_autotune",
+    "mdx_set_thermostat", "mdx_thermostat_state",
 )
 
 _lib = None
@@ -77,6 +78,8 @@ def load():
                                    C.c_int]
     L.mdx_set_pair_config.argtypes = [vp, C.c_int, C.c_int]
     L.mdx_autotune.argtypes = [vp, C.c_int, _ip, C.POINTER(C.c_double)]
+    L.mdx_set_thermostat.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int]
+    L.mdx_thermostat_state.argtypes = [vp, _dp]
     L.mdx_adjoint.argtypes = [vp, C.c_int64] + [vp] * 8
     _lib = L
     return L
@@ -162,6 +165,27 @@ class Context:
         ms = C.c_double(0)
         self._chk(self.L.mdx_autotune(self.h, int(reps), ch, C.byref(ms)), "mdx_autotune")
         return (int(ch[0]), int(ch[1])), ms.value
+
+    def set_thermostat(self, T_target=None, lrc=False, explosion_heuristics=True):
+        """NVT tail of every step (global velocity-scaling thermostat per
+        replica, homogeneous long-range correction); T_target: scalar or one
+        value per replica, None or <= 0 for no thermostat."""
+        nrep = self.nrep
+        if T_target is None:
+            tp = None
+        else:
+            t = np.ascontiguousarray(np.broadcast_to(np.asarray(T_target, np.float64), (nrep,)))
+            tp = t.ctypes.data_as(C.c_void_p)
+        self._chk(self.L.mdx_set_thermostat(self.h, nrep, tp, int(bool(lrc)),
+                                            int(bool(explosion_heuristics))),
+                  "mdx_set_thermostat")
+
+    def thermostat_state(self):
+        """[nrep, 8]: T_target, UpotCorr, VirialCorr, VirialCorr/(3N), T,
+        beta_trans, beta_rot, explosion flag of the last step."""
+        out = np.zeros((self.nrep, 8))
+        self._chk(self.L.mdx_thermostat_state(self.h, out), "mdx_thermostat_state")
+        return out
 
     def setup_replicas(self, systems, periodic=True, precision=PREC_FP64, cell_mode=0):
         """Batched replicas from a list of systems that share components and

@@ -33,6 +33,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
 #define OX_MAXCOMP 16
 #define OX_MAXSITE 8
 
@@ -1084,6 +1087,226 @@ int ox_simulate(void *h, int n, int nsteps, double dt, double *r, double *v, dou
             o[3] = sums[1];
             o[4] = mac[OX_UPOT6LJ] / 6.0;
             o[5] = mac[OX_UPOTXPOLES] + mac[OX_MYRF];
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------- global values, LRC, thermostat */
+
+/* Homogeneous::_TICCu/_TICSu/_TISSu/_TICCv/_TICSv/_TISSv
+ * (src/longRange/Homogeneous.cpp:158-196): radial integrals of the r^(2n)
+ * terms beyond rc for centre-centre, centre-site and site-site distances
+ * tau (orientation averaged). */
+static double ticc_u(int n, double rc, double s2) {
+    return -pow(rc, 2 * n + 3) / (pow(s2, n) * (2 * n + 3));
+}
+static double tics_u(int n, double rc, double s2, double tau) {
+    return -(pow(rc + tau, 2 * n + 3) - pow(rc - tau, 2 * n + 3)) * rc /
+               (4 * pow(s2, n) * tau * (n + 1) * (2 * n + 3)) +
+           (pow(rc + tau, 2 * n + 4) - pow(rc - tau, 2 * n + 4)) /
+               (4 * pow(s2, n) * tau * (n + 1) * (2 * n + 3) * (2 * n + 4));
+}
+static double tiss_u(int n, double rc, double s2, double t1, double t2) {
+    double tp = t1 + t2, tm = t1 - t2;
+    return -(pow(rc + tp, 2 * n + 4) - pow(rc + tm, 2 * n + 4) - pow(rc - tm, 2 * n + 4) +
+             pow(rc - tp, 2 * n + 4)) *
+               rc / (8 * pow(s2, n) * t1 * t2 * (n + 1) * (2 * n + 3) * (2 * n + 4)) +
+           (pow(rc + tp, 2 * n + 5) - pow(rc + tm, 2 * n + 5) - pow(rc - tm, 2 * n + 5) +
+            pow(rc - tp, 2 * n + 5)) /
+               (8 * pow(s2, n) * t1 * t2 * (n + 1) * (2 * n + 3) * (2 * n + 4) * (2 * n + 5));
+}
+static double ticc_v(int n, double rc, double s2) { return 2 * n * ticc_u(n, rc, s2); }
+static double tics_v(int n, double rc, double s2, double tau) {
+    return -(pow(rc + tau, 2 * n + 2) - pow(rc - tau, 2 * n + 2)) * rc * rc /
+               (4 * pow(s2, n) * tau * (n + 1)) -
+           3 * tics_u(n, rc, s2, tau);
+}
+static double tiss_v(int n, double rc, double s2, double t1, double t2) {
+    double tp = t1 + t2, tm = t1 - t2;
+    return -(pow(rc + tp, 2 * n + 3) - pow(rc + tm, 2 * n + 3) - pow(rc - tm, 2 * n + 3) +
+             pow(rc - tp, 2 * n + 3)) *
+               rc * rc / (8 * pow(s2, n) * t1 * t2 * (n + 1) * (2 * n + 3)) -
+           3 * tiss_u(n, rc, s2, t1, t2);
+}
+
+/* Homogeneous::init + calculateLongRange (Homogeneous.cpp:21-146) for nmol[i]
+ * molecules of component i in a box of volume `volume`: out[0] = UpotCorr,
+ * out[1] = VirialCorr, out[2] = the per-molecule virial diagonal addition
+ * VirialCorr / (3 N). Only LJ table entries with shift6 == 0 contribute (the
+ * reference reads uLJshift6 from Comp2Param, so truncated-shifted and all
+ * cross-component pairs add nothing); MySelbstTerm is the reaction-field self
+ * term of the molecules' dipole (charges' first moment + point dipoles).
+ * Returns -1 where the reference exits (tau1 + tau2 >= rcLJ). */
+int ox_lrc_homogeneous(void *h, const double *nmol, double volume, double *out) {
+    ox_ctx *x = (ox_ctx *)h;
+    double upot = 0., vir = 0., self = 0., N = 0.;
+    const int nl = x->nlj_total;
+    for (int i = 0; i < x->ncomp; ++i) {
+        const ox_comp *ci = &x->comp[i];
+        N += nmol[i];
+        double cb[3] = {0, 0, 0};
+        for (int s = 0; s < ci->nc; ++s)
+            for (int d = 0; d < 3; ++d) cb[d] += ci->c_q[s] * ci->c[s][d];
+        for (int s = 0; s < ci->nd; ++s) {
+            double ev = 0.;
+            for (int d = 0; d < 3; ++d) ev += ci->d_e[s][d] * ci->d_e[s][d];
+            double norm = 1.0 / sqrt(ev);
+            for (int d = 0; d < 3; ++d) cb[d] += ci->d_p[s] * ci->d_e[s][d] * norm;
+        }
+        double my2 = 0.;
+        for (int d = 0; d < 3; ++d) my2 += cb[d] * cb[d];
+        self += my2 * nmol[i];
+        for (int j = 0; j < x->ncomp; ++j) {
+            const ox_comp *cj = &x->comp[j];
+            for (int si = 0; si < ci->nlj; ++si) {
+                const double *a = ci->lj[si];
+                double t1 = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+                for (int sj = 0; sj < cj->nlj; ++sj) {
+                    const double *b = cj->lj[sj];
+                    double t2 = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+                    if (t1 + t2 >= x->rcLJ) return -1;
+                    const int k = (ci->lj_base + si) * nl + cj->lj_base + sj;
+                    const double eps24 = x->eps24[k], sig2 = x->sig2[k];
+                    if (x->shift6[k] != 0.0) continue;
+                    double fac = nmol[i] * nmol[j] * eps24;
+                    const double rc = x->rcLJ;
+                    if (t1 == 0. && t2 == 0.) {
+                        upot += fac * (ticc_u(-6, rc, sig2) - ticc_u(-3, rc, sig2));
+                        vir += fac * (ticc_v(-6, rc, sig2) - ticc_v(-3, rc, sig2));
+                    } else if (t1 != 0. && t2 != 0.) {
+                        upot += fac * (tiss_u(-6, rc, sig2, t1, t2) - tiss_u(-3, rc, sig2, t1, t2));
+                        vir += fac * (tiss_v(-6, rc, sig2, t1, t2) - tiss_v(-3, rc, sig2, t1, t2));
+                    } else {
+                        double t = (t2 == 0.) ? t1 : t2;
+                        upot += fac * (tics_u(-6, rc, sig2, t) - tics_u(-3, rc, sig2, t));
+                        vir += fac * (tics_v(-6, rc, sig2, t) - tics_v(-3, rc, sig2, t));
+                    }
+                }
+            }
+        }
+    }
+    const double rho = N / volume;
+    const double fac = M_PI * rho / (3. * N);
+    const double uLJ = fac * upot, wLJ = -fac * vir;
+    const double epsRFInvrc3 =
+        2. * (x->epsRF - 1.) / ((x->rc * x->rc * x->rc) * (2. * x->epsRF + 1.));
+    const double myself = -0.5 * epsRFInvrc3 * self;
+    out[0] = uLJ + myself;
+    out[1] = wLJ + 3. * myself;
+    out[2] = out[1] / (3. * N);
+    return 0;
+}
+
+/* Domain::calculateGlobalValues for the one global thermostat
+ * (Domain.cpp:157-312) with the default temperature schedule Tfactor = 1
+ * (Simulation.cpp:104-105, 1489-1501). in = {summv2, sumIw2, N, rotDOF} from
+ * upd_postF. out = {T, beta_trans, beta_rot, explode}. The explosion
+ * heuristic (Domain.cpp:261-313) fires when a beta is below MIN_BETA = 0.9;
+ * its error counter can never become positive in the reference's bookkeeping
+ * (the counter is only ever set to 4000), so it stays armed. */
+void ox_thermostat_params(const double *in, double Ttarget, int heuristics, double *out) {
+    const double summv2 = in[0], N = in[2], rot = in[3];
+    const double sumIw2 = (rot > 0) ? in[1] : 0.0;
+    out[0] = (N > 0) ? (summv2 + sumIw2) / (3 * N + rot) : Ttarget;
+    double bt = 1.0, br = 1.0;
+    if (Ttarget > 0.0 && N > 0) {
+        bt = pow(3.0 * N * Ttarget / summv2, 0.4);
+        br = (sumIw2 == 0.0) ? 1.0 : pow(rot * Ttarget / sumIw2, 0.4);
+    }
+    int explode = 0;
+    if ((bt < 0.9 || br < 0.9) && heuristics) {
+        explode = 1;
+        bt = 1.0;
+        br = pow(br, 0.0091);
+    }
+    out[1] = bt;
+    out[2] = br;
+    out[3] = explode;
+}
+
+/* The explosion capping (Domain.cpp:263-297: kinetic energies above
+ * 10 Ti translational / 3 rotDOF Ti rotational are scaled down together with
+ * F and M) when tp[3] != 0, then VelocityScalingThermostat::apply without a
+ * directed velocity (VelocityScalingThermostat.cpp:70-90): v *= beta_trans,
+ * D *= beta_rot. tp = ox_thermostat_params output. */
+void ox_velocity_scaling(void *h, int n, double *v, double *L, double *F, double *M,
+                         const double *q, const int *cid, const double *tp, double Ttarget) {
+    ox_ctx *x = (ox_ctx *)h;
+    for (int i = 0; i < n; ++i) {
+        const ox_comp *c = &x->comp[cid[i]];
+        double *vi = v + 3 * i, *Li = L + 3 * i;
+        if (tp[3] != 0.) {
+            const double limit = 10.0 * Ttarget;
+            const double ut = 0.5 * c->m * (vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2]);
+            if (ut > limit) {
+                const double vc = sqrt(limit / ut);
+                for (int d = 0; d < 3; ++d) {
+                    vi[d] *= vc;
+                    F[3 * i + d] *= vc;
+                }
+            }
+            if (c->rotdof > 0) {
+                const double lr = 3.0 * c->rotdof * Ttarget;
+                double w[3], Iw2 = 0.;
+                qrotinv(q + 4 * i, Li, w);
+                for (int d = 0; d < 3; ++d) {
+                    w[d] *= c->invI[d];
+                    Iw2 += c->I[d] * w[d] * w[d];
+                }
+                const double ur = 0.5 * Iw2;
+                if (ur > lr) {
+                    const double dc = sqrt(lr / ur);
+                    for (int d = 0; d < 3; ++d) {
+                        Li[d] *= dc;
+                        M[3 * i + d] *= dc;
+                    }
+                }
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            vi[d] *= tp[1];
+            Li[d] *= tp[2];
+        }
+    }
+}
+
+/* ox_simulate with the NVT tail of Simulation::simulateOneTimestep
+ * (Simulation.cpp:1161-1266): the homogeneous long-range correction adds
+ * lrc[2] to every molecule's virial diagonal after the traversal and lrc[0],
+ * lrc[1] to the global Upot / virial (Domain.cpp:187-188); after upd_postF the
+ * thermostat parameters are computed and the velocities scaled. lrc may be
+ * NULL (no correction); Ttarget <= 0 disables the thermostat (betas 1).
+ * per_step[s*9 + ...] = the six ox_simulate values (Upot and virial including
+ * the correction), then T, beta_trans, beta_rot. */
+int ox_simulate_nvt(void *h, int n, int nsteps, double dt, double *r, double *v, double *q,
+                    double *L, const int *cid, double *F, double *M, double *Vi,
+                    const double *lrc, double Ttarget, int heuristics, double *per_step) {
+    double mac[4], sums[4], tp[4];
+    for (int s = 0; s < nsteps; ++s) {
+        ox_leapfrog_pre(h, n, dt, r, v, q, L, F, M, cid);
+        if (((ox_ctx *)h)->periodic) ox_wrap(h, n, r);
+        if (ox_forces(h, n, r, q, cid, F, M, Vi, mac, NULL)) return -1;
+        if (lrc)
+            for (int i = 0; i < 3 * n; ++i) Vi[i] = lrc[2] + Vi[i];
+        ox_leapfrog_post(h, n, dt, v, L, q, F, M, cid, sums);
+        ox_thermostat_params(sums, Ttarget, heuristics, tp);
+        if (Ttarget > 0.0) ox_velocity_scaling(h, n, v, L, F, M, q, cid, tp, Ttarget);
+        if (per_step) {
+            double *o = per_step + 9 * s;
+            o[0] = mac[OX_UPOT6LJ] / 6.0 + mac[OX_UPOTXPOLES] + mac[OX_MYRF];
+            o[1] = mac[OX_VIRIAL] + 3.0 * mac[OX_MYRF];
+            if (lrc) {
+                o[0] = o[0] + lrc[0];
+                o[1] = o[1] + lrc[1];
+            }
+            o[2] = sums[0];
+            o[3] = sums[1];
+            o[4] = mac[OX_UPOT6LJ] / 6.0;
+            o[5] = mac[OX_UPOTXPOLES] + mac[OX_MYRF];
+            o[6] = tp[0];
+            o[7] = tp[1];
+            o[8] = tp[2];
         }
     }
     return 0;

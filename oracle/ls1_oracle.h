@@ -64,6 +64,16 @@ int ox_simulate(void *h, int n, int nsteps, double dt, double *r, double *v,
                 double *q, double *L, const int *cid, double *F, double *M,
                 double *Vi, double *per_step);
 
+/* global values, homogeneous long-range correction, velocity-scaling
+ * thermostat; see ls1_oracle.c */
+int ox_lrc_homogeneous(void *h, const double *nmol, double volume, double *out);
+void ox_thermostat_params(const double *in, double Ttarget, int heuristics, double *out);
+void ox_velocity_scaling(void *h, int n, double *v, double *L, double *F, double *M,
+                         const double *q, const int *cid, const double *tp, double Ttarget);
+int ox_simulate_nvt(void *h, int n, int nsteps, double dt, double *r, double *v, double *q,
+                    double *L, const int *cid, double *F, double *M, double *Vi,
+                    const double *lrc, double Ttarget, int heuristics, double *per_step);
+
 /* FP64 adjoint (brute force, LJ and charge-charge): parameter gradients of
  * Upot and virial and the force vector-Jacobian product; see ls1_oracle.c */
 void ox_lj_shift_rc(void *h, double *out);

@@ -65,6 +65,13 @@ class Oracle:
             L.ox_simulate.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp,
                                       _dp, _ip, _dp, _dp, _dp, _dp]
             L.ox_set_lj_tables.argtypes = [C.c_void_p, _dp, _dp, _dp]
+            L.ox_lrc_homogeneous.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+            L.ox_thermostat_params.argtypes = [_dp, C.c_double, C.c_int, _dp]
+            L.ox_velocity_scaling.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip,
+                                              _dp, C.c_double]
+            L.ox_simulate_nvt.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp,
+                                          _dp, _ip, _dp, _dp, _dp, C.c_void_p, C.c_double,
+                                          C.c_int, _dp]
             L.ox_lj_shift_rc.argtypes = [C.c_void_p, _dp]
             L.ox_adjoint.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp,
                                      _dp, _dp, _dp]
@@ -197,6 +204,51 @@ class Oracle:
             raise RuntimeError("oracle simulate failed")
         return dict(r=r, v=v, q=q, L=L, F=F, M=M, Vi=Vi, per_step=per.reshape(-1, 6)[:nsteps],
                     initial=f0)
+
+    def lrc_homogeneous(self, nmol, volume):
+        """UpotCorr, VirialCorr, VirialCorr/(3N) of Homogeneous (ox_lrc_homogeneous)."""
+        out = np.zeros(3)
+        if self.lib().ox_lrc_homogeneous(self.h, _f64(nmol), float(volume), out):
+            raise RuntimeError("long-range correction: rcLJ too small")
+        return out
+
+    @classmethod
+    def thermostat_params(cls, sums, T_target, heuristics=True):
+        """T, beta_trans, beta_rot, explode from the upd_postF sums."""
+        out = np.zeros(4)
+        cls.lib().ox_thermostat_params(_f64(sums), float(T_target), int(bool(heuristics)), out)
+        return out
+
+    def velocity_scaling(self, v, L, F, M, q, cid, tp, T_target):
+        v, L, F, M = (_f64(a).copy() for a in (v, L, F, M))
+        self.lib().ox_velocity_scaling(self.h, len(v), v, L, F, M, _f64(q),
+                                       np.ascontiguousarray(cid, np.int32), _f64(tp),
+                                       float(T_target))
+        return v, L, F, M
+
+    def simulate_nvt(self, nsteps, dt, r, v, q, L, cid, T_target, lrc=False, heuristics=True):
+        """ox_simulate with the NVT tail: initial forces (with the long-range
+        virial addition when lrc), then nsteps; per_step rows are Upot,
+        virial, summv2, sumIw2, Upot_LJ, Upot_rest, T, beta_trans, beta_rot."""
+        r, v, q, L = (_f64(a).copy() for a in (r, v, q, L))
+        cid = np.ascontiguousarray(cid, np.int32)
+        terms = None
+        if lrc:
+            nmol = np.bincount(cid, minlength=self.ncomp).astype(np.float64)
+            terms = self.lrc_homogeneous(nmol, float(np.prod(self.box)))
+        f0 = self.forces(r, q, cid)
+        q = f0["q"]
+        F, M, Vi = f0["F"].copy(), f0["M"].copy(), f0["Vi"].copy()
+        if terms is not None:
+            Vi = terms[2] + Vi
+        per = np.zeros(9 * max(nsteps, 1))
+        tp = terms.ctypes.data_as(C.c_void_p) if terms is not None else None
+        rc = self.lib().ox_simulate_nvt(self.h, len(r), int(nsteps), float(dt), r, v, q, L, cid,
+                                        F, M, Vi, tp, float(T_target), int(bool(heuristics)), per)
+        if rc:
+            raise RuntimeError("oracle simulate failed")
+        return dict(r=r, v=v, q=q, L=L, F=F, M=M, Vi=Vi, per_step=per.reshape(-1, 9)[:nsteps],
+                    lrc=terms)
 
     @classmethod
     def site_pair(cls, kind, ri, ei, pi, rj, ej, pj, eps24=0.0, sig2=0.0, shift6=0.0,
