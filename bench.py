@@ -76,7 +76,7 @@ def make_workload(name, scale):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_gpu(s, precision, steps, warmup, kernel_reps):
+def run_gpu(s, precision, steps, warmup, kernel_reps, rmm=False):
     t_start = time.perf_counter()
 
     def phase(msg):
@@ -126,8 +126,21 @@ def run_gpu(s, precision, steps, warmup, kernel_reps):
         per = ctx.step(steps, s.dt, use_graph=True)
         ctx.synchronize()
         t1 = time.perf_counter()
+        rmm_s = None
+        if rmm:
+            # the reduced-memory processor on the same state (half kick, then
+            # r-update / re-bin / force-with-kick / v^2 sums per step)
+            ctx.set_rmm(s.dt)
+            if warmup:
+                ctx.step(warmup, s.dt, use_graph=True)
+            ctx.synchronize()
+            t2 = time.perf_counter()
+            ctx.step(steps, s.dt, use_graph=True)
+            ctx.synchronize()
+            rmm_s = time.perf_counter() - t2
+            phase(f"reduced-memory mode: {rmm_s * 1e3 / steps:.3f} ms/step")
     return dict(seconds=t1 - t0, per_step=per, counters=counters, pair_ms=float(np.mean(pair_ms)),
-                pair_ms_all=pair_ms, init=init, shape=shape)
+                pair_ms_all=pair_ms, init=init, shape=shape, rmm_seconds=rmm_s)
 
 
 def cpu_baseline(workload, budget_s):
@@ -409,7 +422,8 @@ def main():
     forms = {"fp32": PREC_FP32_FP64ACC, "fp64": PREC_FP64}
     for tag in (["fp32", "fp64"] if args.only == "both" else [args.only]):
         log(f"GPU run {tag}: warmup {args.warmup}, timed {args.steps} steps")
-        res[tag] = run_gpu(s, forms[tag], args.steps, args.warmup, args.kernel_reps)
+        res[tag] = run_gpu(s, forms[tag], args.steps, args.warmup, args.kernel_reps,
+                           rmm=args.workload in ("ljts_slab", "lj_fcc"))
         r = res[tag]
         log(f"  {r['seconds'] * 1e3 / args.steps:.3f} ms/step, pair kernel {r['pair_ms']:.3f} ms")
 
@@ -462,6 +476,12 @@ def main():
         out["force_rel_rms_err_fp32_vs_fp64"] = err
         u32, u64 = res["fp32"]["per_step"][:, 0], r64["per_step"][:, 0]
         out["upot_rel_err_first_step"] = float(abs(u32[0] - u64[0]) / abs(u64[0]))
+    for tag in res:
+        if res[tag].get("rmm_seconds"):
+            rs = res[tag]["rmm_seconds"]
+            out.setdefault("reduced_memory_mode", {})[tag] = {
+                "value": s.n * args.steps / rs, "ms_per_step": rs * 1e3 / args.steps,
+                "note": "VCP1CLJRMM + LeapfrogRMM equivalent: kick inside the pair kernel"}
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle, 1 thread)")
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_budget)

@@ -137,6 +137,18 @@ int mdx_autotune(mdx_context *ctx, int reps, int32_t *chosen, double *best_ms);
 int mdx_set_thermostat(mdx_context *ctx, int nrep, const double *T_target, int lrc,
                        int explosion_heuristics);
 
+/* Reduced-memory one-centre mode: the VCP1CLJRMM processor with LeapfrogRMM
+ * (VCP1CLJRMM.cpp:300-362, LeapfrogRMM.cpp:42-107, PositionCellProcessorRMM.h,
+ * VelocityCellProcessorRMM.h). Velocities live at half steps: this call runs
+ * the initial force traversal with dt/m halved (Simulation.cpp:902-937), and
+ * every following mdx_step (same dt) moves r += dt v, re-bins, and lets the
+ * pair kernel add dt/m F to v directly (no F / Vi stored); the step record's
+ * sum m v^2 becomes the reference's sum v^2 (no mass factor) of the half-step
+ * velocities, sum I w^2 = 0. One component of one LJ centre only (as the
+ * reference's reduced-memory build). dt = 0 switches back (velocities stay
+ * where they are); a new upload also ends the mode. */
+int mdx_set_rmm(mdx_context *ctx, double dt);
+
 /* Per-replica record of the last step's NVT tail, out[nrep][8] = {T_target,
  * UpotCorr, VirialCorr, VirialCorr/(3N), T, beta_trans, beta_rot, explosion
  * (0/1)}; T is computed before the velocities are scaled (Domain.cpp:226-230). */

@@ -16,7 +16,7 @@ namespace mdx {
 __global__ void k_leapfrog_pre(MolArrays, const double *, const double *, const Component *, Reps,
                                double, int);
 __global__ void k_leapfrog_post(MolArrays, const double *, const double *, const Component *,
-                                double, int, const int *, const int *, double *);
+                                double, int, const int *, const int *, double *, int);
 __global__ void k_seg_reduce(const double *, int, const int *, double *);
 __global__ void k_reduce_partials(const double *, int, int, int, double *);
 __global__ void k_fill_int(int *, int, int);
@@ -79,6 +79,9 @@ struct mdx_context {
     std::vector<double> T_target;    // [nrep]; <= 0: no thermostat for that replica
     std::vector<double> comp_count;  // [nrep][ncomp] molecules per component
     double *d_thermo = nullptr;      // [nrep][kThermoW]
+    // reduced-memory one-centre mode (VCP1CLJRMM + LeapfrogRMM): velocities at
+    // half steps, the kick inside the pair kernel; 0 = off, else its dt
+    double rmm_dt = 0.0;
     std::vector<double> eps24, sig2, shift6;
     std::vector<double> shift_rc;  // radius of the shift tied to (eps, sigma); 0 = none
     double *d_eps24 = nullptr, *d_sig2 = nullptr, *d_shift6 = nullptr, *d_shift_rc = nullptr;
@@ -716,7 +719,9 @@ int launch_update(mdx_context *c) {
     return MDX_OK;
 }
 
-int launch_traverse(mdx_context *c) {
+// kick != 0 (reduced-memory mode): the pair kernel adds kick * F to the
+// velocities instead of storing F and Vi.
+int launch_traverse(mdx_context *c, double kick = 0.0) {
     PairArgs a;
     memset(&a, 0, sizeof(a));
     a.R = reps(c);
@@ -729,6 +734,13 @@ int launch_traverse(mdx_context *c) {
     }
     a.vscale = c->point_lj ? 0.5 : 1.0;
     a.thermo = (c->lrc && c->point_lj) ? c->d_thermo : nullptr;
+    if (kick != 0.0) {
+        const MolArrays &m = c->mol[c->cur];
+        a.kvx = m.vx;
+        a.kvy = m.vy;
+        a.kvz = m.vz;
+        a.kick = kick;
+    }
     a.pp.eps24 = c->d_eps24;
     a.pp.sig2 = c->d_sig2;
     a.pp.shift6 = c->d_shift6;
@@ -776,8 +788,10 @@ int launch_traverse(mdx_context *c) {
 }
 
 int launch_lf_pre(mdx_context *c, double dt) {
+    // reduced-memory mode: PositionCellProcessorRMM, r += dt v only
     hipLaunchKernelGGL(k_leapfrog_pre, dim3(nblk(c->n)), dim3(kBlock), 0, c->stream,
-                       c->mol[c->cur], c->F, c->M, c->d_comps, reps(c), dt, c->n);
+                       c->mol[c->cur], c->rmm_dt != 0.0 ? nullptr : c->F, c->M, c->d_comps,
+                       reps(c), dt, c->n);
     HIPCHK(c, hipGetLastError());
     return MDX_OK;
 }
@@ -785,7 +799,8 @@ int launch_lf_pre(mdx_context *c, double dt) {
 int launch_lf_post(mdx_context *c, double dt) {
     const int R = c->nrep, nb = c->lf_nchunk;
     hipLaunchKernelGGL(k_leapfrog_post, dim3(nb), dim3(kBlock), 0, c->stream, c->mol[c->cur], c->F,
-                       c->M, c->d_comps, dt, c->n, c->d_lf_chunk, c->d_lf_chunk_end, c->lf_part);
+                       c->M, c->d_comps, dt, c->n, c->d_lf_chunk, c->d_lf_chunk_end, c->lf_part,
+                       c->rmm_dt != 0.0 ? 1 : 0);
     if (R == 1)
         launch_reduce(c, c->lf_part, nb, 4, c->red + 8);
     else
@@ -804,7 +819,10 @@ int launch_lf_post(mdx_context *c, double dt) {
 int launch_step(mdx_context *c, double dt) {
     TRY(launch_lf_pre(c, dt));
     TRY(launch_update(c));
-    TRY(launch_traverse(c));
+    if (c->rmm_dt != 0.0)  // LeapfrogRMM: one full kick dt/m inside the force kernel
+        TRY(launch_traverse(c, dt / c->model.comp[0].m));
+    else
+        TRY(launch_traverse(c));
     TRY(launch_lf_post(c, dt));
     hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, c->stream, c->step_idx);
     HIPCHK(c, hipGetLastError());
@@ -838,6 +856,7 @@ int state_setup(mdx_context *c, int64_t n64, const int32_t *cid_host) {
     }
     const int n = (int)n64;
     drop_graphs(c);
+    c->rmm_dt = 0.0;  // a new state starts with full-step velocities
     HIPCHK(c, hipStreamSynchronize(c->stream));
     free_list(c->state_allocs);
     c->n = n;
@@ -1115,6 +1134,29 @@ int mdx_set_thermostat(mdx_context *c, int nrep, const double *T_target, int lrc
     return upload_thermo(c);
 }
 
+int mdx_set_rmm(mdx_context *c, double dt) {
+    if (!c) return MDX_ERR_ARG;
+    TRY(need(c, true, true, true));
+    if (dt == 0.0) {
+        c->rmm_dt = 0.0;
+        drop_graphs(c);
+        return MDX_OK;
+    }
+    if (!(dt > 0.0)) return fail(c, MDX_ERR_ARG, "dt must be positive");
+    if (c->model.ncomp != 1 || !c->point_lj)
+        return fail(c, MDX_ERR_ARG, "reduced-memory mode: one component of one LJ centre only");
+    if (c->rmm_dt != 0.0) return fail(c, MDX_ERR_STATE, "reduced-memory mode is already on");
+    HIPCHK(c, hipSetDevice(c->device));
+    drop_graphs(c);
+    // Simulation::prepare_start (Simulation.cpp:902-937): the initial force
+    // traversal with dt/m halved moves the velocities to the half step
+    TRY(launch_update(c));
+    TRY(launch_traverse(c, 0.5 * (dt / c->model.comp[0].m)));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->rmm_dt = dt;
+    return MDX_OK;
+}
+
 int mdx_thermostat_state(mdx_context *c, double *out) {
     if (!c || !out) return MDX_ERR_ARG;
     TRY(need(c, true, true, false));
@@ -1369,6 +1411,8 @@ int mdx_leapfrog_post(mdx_context *c, double dt, double *sums) {
 int mdx_step(mdx_context *c, int nsteps, double dt, double *per_step, int use_graph) {
     if (!c || nsteps < 0) return MDX_ERR_ARG;
     TRY(need(c, true, true, true));
+    if (c->rmm_dt != 0.0 && dt != c->rmm_dt)
+        return fail(c, MDX_ERR_ARG, "reduced-memory mode: dt differs from mdx_set_rmm's");
     HIPCHK(c, hipSetDevice(c->device));
     TRY(ensure_steps(c, nsteps));
     HIPCHK(c, hipMemsetAsync(c->step_idx, 0, sizeof(int), c->stream));

@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_leapfrog_pre(MolArrays s, const double 
     double r[3] = {s.rx[i], s.ry[i], s.rz[i]};
     double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
     for (int d = 0; d < 3; ++d) {
-        v[d] += dtInv2m * F[d * n + i];
+        if (F) v[d] += dtInv2m * F[d * n + i];  // F null: PositionCellProcessorRMM, r only
         r[d] += dt * v[d];
     }
     if (!c.point) rotate_pre(s, M, c, dt, n, i);
@@ -184,11 +184,18 @@ __global__ __launch_bounds__(256) void k_leapfrog_post(MolArrays s, const double
                                                        double dt, int n,
                                                        const int *__restrict__ chunk,
                                                        const int *__restrict__ chunk_end,
-                                                       double *partial) {
+                                                       double *partial, int rmm) {
     __shared__ double red[4][4];
     int i = chunk[blockIdx.x] + threadIdx.x;
     double mv2 = 0, iw2 = 0, cnt = 0, rot = 0;
-    if (i < chunk_end[blockIdx.x]) {
+    if (rmm && i < chunk_end[blockIdx.x]) {
+        // VelocityCellProcessorRMM (VelocityCellProcessorRMM.h:60-110): the
+        // kick happened in the force kernel; sum v^2 (no mass factor, as the
+        // reference's reduced-memory mode sums it) and N, no rotation
+        const double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
+        mv2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        cnt = 1.0;
+    } else if (i < chunk_end[blockIdx.x]) {
         const Component &c = comps[s.cid[i]];
         double dt_halve = 0.5 * dt;
         double dtInv2m = dt_halve / c.m;

@@ -436,7 +436,13 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
                     acc.v[k] += shfl_xor_d(acc.v[k], o);
                     if constexpr (TI >= T_D) acc.m[k] += shfl_xor_d(acc.m[k], o);
                 }
-            if (valid && ph == 0) {
+            if (valid && ph == 0 && a.kvx) {
+                // VCP1CLJRMM (VCP1CLJRMM.cpp:300-362): the site's force, times
+                // dt/m, goes straight into its molecule's velocity
+                a.kvx[i] = __dadd_rn(a.kvx[i], __dmul_rn(acc.f[0], a.kick));
+                a.kvy[i] = __dadd_rn(a.kvy[i], __dmul_rn(acc.f[1], a.kick));
+                a.kvz[i] = __dadd_rn(a.kvz[i], __dmul_rn(acc.f[2], a.kick));
+            } else if (valid && ph == 0) {
                 S.Fx[i] = acc.f[0]; S.Fy[i] = acc.f[1]; S.Fz[i] = acc.f[2];
                 double vv[3] = {__dmul_rn(a.vscale, acc.v[0]), __dmul_rn(a.vscale, acc.v[1]),
                                 __dmul_rn(a.vscale, acc.v[2])};

@@ -30,7 +30,7 @@ EXPORTS = (
     "mdx_cell_ids", "mdx_cell_geometry", "mdx_cell_pairs", "mdx_set_counting", "mdx_counters",
     "mdx_upload_device", "mdx_set_timing", "mdx_last_kernel_ms", "mdx_set_charges",
     "mdx_adjoint", "mdx_lj_tables", "mdx_set_replicas", "mdx_set_pair_config", "mdx_autotune",
-    "mdx_set_thermostat", "mdx_thermostat_state",
+    "mdx_set_thermostat", "mdx_thermostat_state", "mdx_set_rmm",
 )
 
 _lib = None
@@ -80,6 +80,7 @@ def load():
     L.mdx_autotune.argtypes = [vp, C.c_int, _ip, C.POINTER(C.c_double)]
     L.mdx_set_thermostat.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int]
     L.mdx_thermostat_state.argtypes = [vp, _dp]
+    L.mdx_set_rmm.argtypes = [vp, C.c_double]
     L.mdx_adjoint.argtypes = [vp, C.c_int64] + [vp] * 8
     _lib = L
     return L
@@ -179,6 +180,11 @@ class Context:
         self._chk(self.L.mdx_set_thermostat(self.h, nrep, tp, int(bool(lrc)),
                                             int(bool(explosion_heuristics))),
                   "mdx_set_thermostat")
+
+    def set_rmm(self, dt):
+        """Reduced-memory one-centre mode (half-step velocities, kick in the
+        force kernel); runs the initial half kick. dt = 0 switches it off."""
+        self._chk(self.L.mdx_set_rmm(self.h, float(dt)), "mdx_set_rmm")
 
     def thermostat_state(self):
         """[nrep, 8]: T_target, UpotCorr, VirialCorr, VirialCorr/(3N), T,

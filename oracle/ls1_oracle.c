@@ -1312,6 +1312,60 @@ int ox_simulate_nvt(void *h, int n, int nsteps, double dt, double *r, double *v,
     return 0;
 }
 
+/* Reduced-memory one-centre mode: VCP1CLJRMM with LeapfrogRMM
+ * (VCP1CLJRMM.cpp:300-362, LeapfrogRMM.cpp:42-107, Simulation.cpp:902-937).
+ * Velocities live at half steps: the initial traversal adds (dt/m)/2 F(r0)
+ * to v; each step moves r += dt v (PositionCellProcessorRMM), wraps, and the
+ * force traversal adds dt/m F to v; VelocityCellProcessorRMM then sums v^2
+ * (no mass factor). One component of one LJ centre. per_step rows as
+ * ox_simulate: Upot, virial, sum v^2, 0, Upot_LJ, Upot_rest. */
+int ox_simulate_rmm(void *h, int n, int nsteps, double dt, double *r, double *v, double *q,
+                    const int *cid, double *per_step) {
+    ox_ctx *x = (ox_ctx *)h;
+    if (x->ncomp != 1 || x->comp[0].nlj != 1 || x->comp[0].nc || x->comp[0].nd ||
+        x->comp[0].nq)
+        return -2;
+    double *F = (double *)calloc(3 * (size_t)n, sizeof(double));
+    double *M = (double *)calloc(3 * (size_t)n, sizeof(double));
+    double *Vi = (double *)calloc(3 * (size_t)n, sizeof(double));
+    double mac[4];
+    int rc = 0;
+    const double dtInvm = dt / x->comp[0].m;
+    if (ox_forces(h, n, r, q, cid, F, M, Vi, mac, NULL)) rc = -1;
+    const double half = dtInvm * 0.5;
+    for (int i = 0; !rc && i < 3 * n; ++i) v[i] = v[i] + F[i] * half;
+    for (int s = 0; !rc && s < nsteps; ++s) {
+        for (int i = 0; i < 3 * n; ++i) r[i] = r[i] + dt * v[i];
+        if (x->periodic) ox_wrap(h, n, r);
+        if (ox_forces(h, n, r, q, cid, F, M, Vi, mac, NULL)) {
+            rc = -1;
+            break;
+        }
+        double summv2 = 0.;
+        for (int i = 0; i < n; ++i) {
+            double v2 = 0.;
+            for (int d = 0; d < 3; ++d) {
+                v[3 * i + d] = v[3 * i + d] + F[3 * i + d] * dtInvm;
+                v2 += v[3 * i + d] * v[3 * i + d];
+            }
+            summv2 += v2;
+        }
+        if (per_step) {
+            double *o = per_step + 6 * s;
+            o[0] = mac[OX_UPOT6LJ] / 6.0 + mac[OX_UPOTXPOLES] + mac[OX_MYRF];
+            o[1] = mac[OX_VIRIAL] + 3.0 * mac[OX_MYRF];
+            o[2] = summv2;
+            o[3] = 0.0;
+            o[4] = mac[OX_UPOT6LJ] / 6.0;
+            o[5] = mac[OX_UPOTXPOLES] + mac[OX_MYRF];
+        }
+    }
+    free(F);
+    free(M);
+    free(Vi);
+    return rc;
+}
+
 /* ------------------------------------------------------------------ adjoint */
 
 /* How the LJ table entry (a,b) depends on its own (eps_ab, sigma_ab):

@@ -74,6 +74,11 @@ int ox_simulate_nvt(void *h, int n, int nsteps, double dt, double *r, double *v,
                     double *L, const int *cid, double *F, double *M, double *Vi,
                     const double *lrc, double Ttarget, int heuristics, double *per_step);
 
+/* reduced-memory one-centre mode (half-step velocities, kick in the force
+ * traversal); see ls1_oracle.c */
+int ox_simulate_rmm(void *h, int n, int nsteps, double dt, double *r, double *v, double *q,
+                    const int *cid, double *per_step);
+
 /* FP64 adjoint (brute force, LJ and charge-charge): parameter gradients of
  * Upot and virial and the force vector-Jacobian product; see ls1_oracle.c */
 void ox_lj_shift_rc(void *h, double *out);

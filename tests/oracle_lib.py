@@ -69,6 +69,8 @@ class Oracle:
             L.ox_thermostat_params.argtypes = [_dp, C.c_double, C.c_int, _dp]
             L.ox_velocity_scaling.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip,
                                               _dp, C.c_double]
+            L.ox_simulate_rmm.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp,
+                                          _ip, _dp]
             L.ox_simulate_nvt.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp,
                                           _dp, _ip, _dp, _dp, _dp, C.c_void_p, C.c_double,
                                           C.c_int, _dp]
@@ -249,6 +251,17 @@ class Oracle:
             raise RuntimeError("oracle simulate failed")
         return dict(r=r, v=v, q=q, L=L, F=F, M=M, Vi=Vi, per_step=per.reshape(-1, 9)[:nsteps],
                     lrc=terms)
+
+    def simulate_rmm(self, nsteps, dt, r, v, q, cid):
+        """Reduced-memory mode: initial half kick, then nsteps; returns the
+        half-step velocities and the per-step records (sum v^2 in column 2)."""
+        r, v, q = (_f64(a).copy() for a in (r, v, q))
+        cid = np.ascontiguousarray(cid, np.int32)
+        per = np.zeros(6 * max(nsteps, 1))
+        rc = self.lib().ox_simulate_rmm(self.h, len(r), int(nsteps), float(dt), r, v, q, cid, per)
+        if rc:
+            raise RuntimeError("oracle simulate_rmm failed")
+        return dict(r=r, v=v, per_step=per.reshape(-1, 6)[:nsteps])
 
     @classmethod
     def site_pair(cls, kind, ri, ei, pi, rj, ej, pj, eps24=0.0, sig2=0.0, shift6=0.0,
