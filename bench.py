@@ -51,15 +51,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(workload, scale, tag):
+def measured_traffic(workload, scale, tag, shape):
     """HBM bytes per pair-kernel launch from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated), or
-    None when no summary covers it."""
+    FETCH_SIZE / WRITE_SIZE summary of this workload and launch shape
+    (copy-calibrated), or None when no summary covers it."""
     if workload != "ljts_slab" or scale != 1.0 or tag != "fp32":
         return None
     f = ROOT / "profiles" / "k_pairs_lj_ljts_slab_3.96M_traffic.json"
     try:
-        return json.loads(f.read_text())["traffic_bytes"]
+        return json.loads(f.read_text())["by_shape"]["%d,%d" % tuple(shape)]["traffic_bytes"]
     except (OSError, ValueError, KeyError):
         return None
 
@@ -455,7 +455,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
-                     "traffic": measured_traffic(args.workload, args.scale, main_tag),
+                     "traffic": measured_traffic(args.workload, args.scale, main_tag, r["shape"]),
                      "kernel": "k_pairs<LJ>", "flops_per_launch": flops,
                      "kernel_ms": r["pair_ms"]},
         "launch_shape": {"cells_per_block": r["shape"][0], "blocks_per_xcd": r["shape"][1]},

@@ -145,8 +145,11 @@ int mdx_set_thermostat(mdx_context *ctx, int nrep, const double *T_target, int l
  * pair kernel add dt/m F to v directly (no F / Vi stored); the step record's
  * sum m v^2 becomes the reference's sum v^2 (no mass factor) of the half-step
  * velocities, sum I w^2 = 0. One component of one LJ centre only (as the
- * reference's reduced-memory build). dt = 0 switches back (velocities stay
- * where they are); a new upload also ends the mode. */
+ * reference's reduced-memory build). While the mode is on, mdx_traverse is
+ * VCP1CLJRMM's traversal too (force and kick), and mdx_leapfrog_pre /
+ * mdx_leapfrog_post are LeapfrogRMM's position / velocity-sum halves.
+ * dt = 0 switches back (velocities stay where they are); a new upload also
+ * ends the mode. */
 int mdx_set_rmm(mdx_context *ctx, double dt);
 
 /* Per-replica record of the last step's NVT tail, out[nrep][8] = {T_target,

@@ -1369,7 +1369,8 @@ int mdx_traverse(mdx_context *c, double *macro) {
         HIPCHK(c, hipMemsetAsync(c->d_counters, 0, 8 * sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipMemsetAsync(c->step_idx, 0, sizeof(int), c->stream));
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    TRY(launch_traverse(c));
+    // in the reduced-memory mode the traversal is VCP1CLJRMM's: force + kick
+    TRY(launch_traverse(c, c->rmm_dt != 0.0 ? c->rmm_dt / c->model.comp[0].m : 0.0));
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     if (macro) {
         TRY(0);

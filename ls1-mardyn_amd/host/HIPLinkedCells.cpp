@@ -213,6 +213,14 @@ void HIPLeapfrog::eventForcesCalculated(ParticleContainer *molCont, Domain *doma
     if (mdx_leapfrog_post(c->context(), _timestepLength, sums) != MDX_OK)
         throw std::runtime_error(std::string("mdx_leapfrog_post: ") + mdx_last_error(c->context()));
     domain->setLocalSums(sums[0], sums[1], (uint64_t)sums[2], (uint64_t)sums[3]);
+    // Domain::calculateGlobalValues and the velocity scaling ran on the device
+    // (when mdx_set_thermostat enabled them); mirror their results
+    double th[8];
+    if (mdx_thermostat_state(c->context(), th) != MDX_OK)
+        throw std::runtime_error(std::string("mdx_thermostat_state: ") + mdx_last_error(c->context()));
+    domain->setUpotCorr(th[1]);
+    domain->setVirialCorr(th[2]);
+    domain->setGlobalBetas(th[5], th[6]);
 }
 
 }  // namespace ls1amd

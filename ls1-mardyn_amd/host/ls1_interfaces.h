@@ -60,7 +60,9 @@ protected:
 };
 
 // Macroscopic sinks and global values (Domain.cpp:115-119, 142-146, 157-270;
-// single process, one thermostat, no long-range correction).
+// single process, one global thermostat). The long-range corrections and the
+// thermostat's factors are computed on the device inside the step and set
+// here by the integrator adapter.
 class Domain {
 public:
     explicit Domain(const double box[3]) {
@@ -87,13 +89,24 @@ public:
     // p = rho T + rho <virial> / 3   (Domain::getGlobalPressure, Domain.cpp:142-146)
     double getGlobalPressure() const {
         double rho = getGlobalRho();
-        return getGlobalTemperature() * rho + rho * (_n ? _localVirial / _n : 0.0) / 3.;
+        return getGlobalTemperature() * rho + rho * (_n ? getGlobalVirial() / _n : 0.0) / 3.;
     }
-    double getAverageGlobalUpot() const { return _n ? _localUpot / _n : 0.0; }
+    // Domain::setUpotCorr / setVirialCorr; global = local + correction
+    // (Domain.cpp:187-188)
+    void setUpotCorr(double u) { _upotCorr = u; }
+    void setVirialCorr(double w) { _virialCorr = w; }
+    double getGlobalUpot() const { return _localUpot + _upotCorr; }
+    double getGlobalVirial() const { return _localVirial + _virialCorr; }
+    double getAverageGlobalUpot() const { return _n ? getGlobalUpot() / _n : 0.0; }
+    // the global thermostat's factors of the last step (Domain.cpp:247-259)
+    void setGlobalBetas(double bt, double br) { _betaTrans = bt; _betaRot = br; }
+    double getGlobalBetaTrans() const { return _betaTrans; }
+    double getGlobalBetaRot() const { return _betaRot; }
 
 private:
     double _box[3];
     double _localUpot = 0, _localVirial = 0, _summv2 = 0, _sumIw2 = 0;
+    double _upotCorr = 0, _virialCorr = 0, _betaTrans = 1, _betaRot = 1;
     uint64_t _n = 0, _rotDOF = 0;
 };
 

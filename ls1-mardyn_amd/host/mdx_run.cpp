@@ -9,7 +9,7 @@
 //
 //   mdx_run <file.inp> --cutoff RC [--lj-cutoff RCLJ] [--dt DT] [--steps N]
 //           [--precision fp64|fp32] [--open-boundary] [--checkpoint OUT.inp]
-//           [--read-write-only OUT.inp]
+//           [--read-write-only OUT.inp] [--nvt T] [--lrc] [--rmm]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -25,7 +25,8 @@ static void usage() {
     std::fprintf(stderr,
                  "usage: mdx_run <file.inp> --cutoff RC [--lj-cutoff RCLJ] [--dt DT] "
                  "[--steps N] [--precision fp64|fp32] [--open-boundary] "
-                 "[--checkpoint OUT.inp] [--read-write-only OUT.inp]\n");
+                 "[--checkpoint OUT.inp] [--read-write-only OUT.inp] [--nvt T] [--lrc] "
+                 "[--rmm]\n");
 }
 
 int main(int argc, char **argv) {
@@ -37,7 +38,8 @@ int main(int argc, char **argv) {
     double rc = -1, rclj = -1, dt = 0.002;
     long steps = 0;
     int precision = MDX_PREC_FP64;
-    bool periodic = true;
+    bool periodic = true, lrc = false, rmm = false;
+    double Ttarget = 0.0;
     for (int i = 2; i < argc; ++i) {
         std::string a = argv[i];
         auto val = [&]() -> std::string {
@@ -55,6 +57,9 @@ int main(int argc, char **argv) {
         else if (a == "--open-boundary") periodic = false;
         else if (a == "--checkpoint") checkpoint = val();
         else if (a == "--read-write-only") rwonly = val();
+        else if (a == "--nvt") Ttarget = std::stod(val());
+        else if (a == "--lrc") lrc = true;
+        else if (a == "--rmm") rmm = true;
         else {
             usage();
             return 1;
@@ -76,11 +81,27 @@ int main(int argc, char **argv) {
         Domain domain(ps.box);
         HIPCellProcessor proc(domain, rc, rclj);
         HIPLeapfrog integrator(dt);
+        // <thermostat type="VelocityScaling"> / <longrange type="homogeneous">
+        if (Ttarget > 0 || lrc) {
+            if (mdx_set_thermostat(cells.context(), 1, &Ttarget, lrc ? 1 : 0, 1) != MDX_OK)
+                throw std::runtime_error(mdx_last_error(cells.context()));
+        }
         cells.addParticles(ps.molecules);
         domain.setglobalNumMolecules(ps.molecules.size());
         // Simulation::prepare_start (Simulation.cpp:847-999)
         cells.update();
         cells.traverseCells(proc);
+        if (rmm) {  // reduced-memory build: the initial half kick (Simulation.cpp:902-937)
+            if (mdx_set_rmm(cells.context(), dt) != MDX_OK)
+                throw std::runtime_error(mdx_last_error(cells.context()));
+        }
+        {  // long-range corrections of the initial configuration
+            double th[8];
+            if (mdx_thermostat_state(cells.context(), th) != MDX_OK)
+                throw std::runtime_error(mdx_last_error(cells.context()));
+            domain.setUpotCorr(th[1]);
+            domain.setVirialCorr(th[2]);
+        }
         std::cout.precision(12);
         std::cout << "Simstep = 0\tU_pot = " << domain.getAverageGlobalUpot()
                   << "\tvirial = " << domain.getLocalVirial() << std::endl;

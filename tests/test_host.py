@@ -54,3 +54,48 @@ def test_driver_trajectory_vs_oracle(tmp_path):
     d = fin.r - ref["r"]
     d -= np.round(d / s.box) * s.box
     assert np.abs(d).max() < 1e-9
+
+
+@pytest.mark.gpu
+def test_driver_nvt_lrc_vs_oracle(tmp_path):
+    """mdx_run --nvt T --lrc: the host layer's step loop with the device's NVT
+    tail against the oracle's NVT step order (global Upot per molecule incl.
+    the correction, final positions)."""
+    s = scenarios.lj_fcc(ncell=4, T=1.0, seed=72)
+    f = tmp_path / "lj.inp"
+    write_inp(f, s.box, s.ncomp, s.comp_desc, s.mix, s.eps_rf, np.arange(1, s.n + 1), s.cid, s.r,
+              s.v, s.q, s.L)
+    r = subprocess.run([str(MDX_RUN), str(f), "--cutoff", str(s.rc), "--dt", "0.002", "--steps",
+                        "20", "--nvt", "0.9", "--lrc", "--checkpoint", str(tmp_path / "out.inp")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    upot = [float(m) for m in re.findall(r"U_pot = ([-0-9.eE+]+)", r.stdout)][1:]
+    temp = [float(m) for m in re.findall(r"T = ([-0-9.eE+]+)", r.stdout)]
+    o = Oracle(1, s.comp_desc, None, 1.0, s.rc, s.rc, s.box, periodic=True)
+    ref = o.simulate_nvt(20, 0.002, s.r, s.v, s.q, s.L, s.cid, 0.9, lrc=True)
+    np.testing.assert_allclose(upot, ref["per_step"][:, 0] / s.n, rtol=1e-9)
+    np.testing.assert_allclose(temp, ref["per_step"][:, 6], rtol=1e-9)
+    fin = read_inp(tmp_path / "out.inp")
+    d = fin.r - ref["r"]
+    d -= np.round(d / s.box) * s.box
+    assert np.abs(d).max() < 1e-9
+
+
+@pytest.mark.gpu
+def test_driver_rmm_vs_oracle(tmp_path):
+    """mdx_run --rmm: the reduced-memory processor through the host layer."""
+    s = scenarios.lj_fcc(ncell=4, T=0.8, seed=73, shifted=True)
+    f = tmp_path / "lj.inp"
+    write_inp(f, s.box, s.ncomp, s.comp_desc, s.mix, s.eps_rf, np.arange(1, s.n + 1), s.cid, s.r,
+              s.v, s.q, s.L)
+    r = subprocess.run([str(MDX_RUN), str(f), "--cutoff", str(s.rc), "--dt", "0.002", "--steps",
+                        "20", "--rmm", "--checkpoint", str(tmp_path / "out.inp")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    o = Oracle(1, s.comp_desc, None, 1.0, s.rc, s.rc, s.box, periodic=True)
+    ref = o.simulate_rmm(20, 0.002, s.r, s.v, s.q, s.cid)
+    fin = read_inp(tmp_path / "out.inp")
+    d = fin.r - ref["r"]
+    d -= np.round(d / s.box) * s.box
+    assert np.abs(d).max() < 1e-9
+    np.testing.assert_allclose(fin.v, ref["v"], rtol=0, atol=1e-9)
