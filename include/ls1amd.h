@@ -45,6 +45,11 @@ extern "C" {
 #define MDX_ERR_NODEVICE (-5) /* no HIP device */
 
 /* precision forms */
+/* boundary types per dimension (BoundaryUtils::BoundaryType,
+ * src/parallel/boundaries/BoundaryUtils.h:29) */
+#define MDX_BOUNDARY_PERIODIC 0
+#define MDX_BOUNDARY_OUTFLOW 1
+#define MDX_BOUNDARY_REFLECTING 2
 #define MDX_PREC_FP64 0       /* all arithmetic FP64 (reference PRECISION=DOUBLE) */
 #define MDX_PREC_FP32_FP64ACC 1 /* FP32 pair arithmetic, FP64 accumulation,
                                    FP64 sums and FP64 Leapfrog (PRECISION=MIXED) */
@@ -136,6 +141,17 @@ int mdx_autotune(mdx_context *ctx, int reps, int32_t *chosen, double *best_ms);
  * Settings are reset by mdx_set_domain / mdx_set_replicas. */
 int mdx_set_thermostat(mdx_context *ctx, int nrep, const double *T_target, int lrc,
                        int explosion_heuristics);
+
+/* Boundary type per dimension, replacing the <boundaries> section
+ * (Simulation.cpp:429-444) after mdx_set_domain / mdx_set_replicas:
+ * MDX_BOUNDARY_PERIODIC (images across the box, wrap), or
+ * MDX_BOUNDARY_REFLECTING: no images in that dimension (the reference deletes
+ * those halos, BoundaryHandler::removeNonPeriodicHalos) and, before every
+ * upd_preF, BoundaryHandler::processGlobalWallLeavingParticles
+ * (BoundaryHandler.cpp:82-134) reverses the velocity of a molecule within rc
+ * of the wall that would cross it. MDX_BOUNDARY_OUTFLOW (deleting leaving
+ * molecules) is rejected: the molecule count is fixed per upload. */
+int mdx_set_boundaries(mdx_context *ctx, const int32_t *types);
 
 /* Reduced-memory one-centre mode: the VCP1CLJRMM processor with LeapfrogRMM
  * (VCP1CLJRMM.cpp:300-362, LeapfrogRMM.cpp:42-107, PositionCellProcessorRMM.h,

@@ -380,7 +380,11 @@ int make_geom(mdx_context *c, const double *box, double rc, int periodic, int ce
         ncell *= nb;
     }
     if (ncell > (1L << 28)) return fail(c, MDX_ERR_CAPACITY, "too many cells");
-    g.periodic = periodic ? 1 : 0;
+    for (int d = 0; d < 3; ++d) {
+        g.pdim[d] = periodic ? 1 : 0;
+        g.reflect[d] = 0;
+    }
+    g.wall_rc = rc;
     ncell_out = ncell;
     return MDX_OK;
 }
@@ -1134,6 +1138,34 @@ int mdx_set_thermostat(mdx_context *c, int nrep, const double *T_target, int lrc
     return upload_thermo(c);
 }
 
+int mdx_set_boundaries(mdx_context *c, const int32_t *types) {
+    if (!c || !types) return MDX_ERR_ARG;
+    TRY(need(c, true, true, false));
+    for (int d = 0; d < 3; ++d) {
+        if (types[d] == MDX_BOUNDARY_OUTFLOW)
+            return fail(c, MDX_ERR_ARG, "outflow boundaries (molecule deletion) are not supported");
+        if (types[d] != MDX_BOUNDARY_PERIODIC && types[d] != MDX_BOUNDARY_REFLECTING)
+            return fail(c, MDX_ERR_ARG, "unknown boundary type");
+    }
+    if (c->rmm_dt != 0.0)
+        return fail(c, MDX_ERR_STATE, "reflecting walls need the stored forces (not in reduced-memory mode)");
+    HIPCHK(c, hipSetDevice(c->device));
+    for (Geom *g : {&c->g}) {
+        for (int d = 0; d < 3; ++d) {
+            g->pdim[d] = types[d] == MDX_BOUNDARY_PERIODIC;
+            g->reflect[d] = types[d] == MDX_BOUNDARY_REFLECTING;
+        }
+    }
+    for (Geom &g : c->geoms)
+        for (int d = 0; d < 3; ++d) {
+            g.pdim[d] = c->g.pdim[d];
+            g.reflect[d] = c->g.reflect[d];
+        }
+    HIPCHK(c, hipMemcpy(c->d_geoms, c->geoms.data(), sizeof(Geom) * c->nrep, hipMemcpyHostToDevice));
+    drop_graphs(c);
+    return MDX_OK;
+}
+
 int mdx_set_rmm(mdx_context *c, double dt) {
     if (!c) return MDX_ERR_ARG;
     TRY(need(c, true, true, true));
@@ -1145,6 +1177,9 @@ int mdx_set_rmm(mdx_context *c, double dt) {
     if (!(dt > 0.0)) return fail(c, MDX_ERR_ARG, "dt must be positive");
     if (c->model.ncomp != 1 || !c->point_lj)
         return fail(c, MDX_ERR_ARG, "reduced-memory mode: one component of one LJ centre only");
+    for (int d = 0; d < 3; ++d)
+        if (c->g.reflect[d])
+            return fail(c, MDX_ERR_STATE, "reduced-memory mode stores no forces for reflecting walls");
     if (c->rmm_dt != 0.0) return fail(c, MDX_ERR_STATE, "reduced-memory mode is already on");
     HIPCHK(c, hipSetDevice(c->device));
     drop_graphs(c);
@@ -1353,11 +1388,8 @@ int mdx_update_container(mdx_context *c) {
     if (!c) return MDX_ERR_ARG;
     TRY(need(c, true, true, true));
     HIPCHK(c, hipSetDevice(c->device));
-    if (c->g.periodic) {
-        // wrap positions that were uploaded outside [0, L) (leapfrog_pre wraps
-        // during stepping): a zero-length Leapfrog drift is not applied here;
-        // molecules must be uploaded inside the box.
-    }
+    // molecules must be uploaded inside the box; leapfrog_pre wraps them
+    // during stepping
     return launch_update(c);
 }
 

@@ -48,7 +48,9 @@ struct Geom {
     double cl[3], rcl[3], halo_min[3], halo_max[3];
     int n[3];          // inner cells per dimension
     int cpd[3];        // ls1 cells per dimension (incl. halo)
-    int periodic;
+    int pdim[3];       // periodic per dimension (images across the box)
+    int reflect[3];    // reflecting walls per dimension (BoundaryType::REFLECTING)
+    double wall_rc;    // width of the wall slab checked for leaving molecules (rc)
     double rc2, rclj2, epsRFInvrc3;
 };
 

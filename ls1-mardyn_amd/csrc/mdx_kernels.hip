@@ -152,13 +152,37 @@ __global__ __launch_bounds__(256) void k_leapfrog_pre(MolArrays s, const double 
     double dtInv2m = dt_halve / c.m;
     double r[3] = {s.rx[i], s.ry[i], s.rz[i]};
     double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
+    if (F && (g.reflect[0] | g.reflect[1] | g.reflect[2])) {
+        // BoundaryHandler::processGlobalWallLeavingParticles, reflecting walls
+        // (BoundaryHandler.cpp:82-134; called before upd_preF,
+        // Simulation.cpp:1114), walls in the order +x, -x, +y, -y, +z, -z: a
+        // molecule in the wall's slab of width rc (RegionUtils.cpp:11-45,
+        // inBox l <= r < u) whose predicted position r + dt (v + dt/2m F)
+        // reaches the wall (RegionUtils.cpp:47-56) gets v = -v - 2 dt/2m F,
+        // so the coming half kick leaves it moving away at -(v + dt/2m F)
+        bool inside = true;
+        for (int d = 0; d < 3; ++d) inside = inside && r[d] >= g.bbmin[d] && r[d] < g.bbmax[d];
+        for (int d = 0; d < 3 && inside; ++d) {
+            if (!g.reflect[d]) continue;
+            const double adj = dtInv2m * F[d * n + i];
+            // +wall: slab [max(max - rc, min), max)
+            if (r[d] >= fmax(g.bbmax[d] - g.wall_rc, g.bbmin[d]) &&
+                r[d] + dt * (v[d] + adj) >= g.bbmax[d])
+                v[d] = -v[d] - adj - adj;
+            // -wall: slab [min, min(min + rc, max))
+            if (r[d] < fmin(g.bbmin[d] + g.wall_rc, g.bbmax[d]) &&
+                r[d] + dt * (v[d] + adj) <= g.bbmin[d])
+                v[d] = -v[d] - adj - adj;
+        }
+    }
     for (int d = 0; d < 3; ++d) {
         if (F) v[d] += dtInv2m * F[d * n + i];  // F null: PositionCellProcessorRMM, r only
         r[d] += dt * v[d];
     }
     if (!c.point) rotate_pre(s, M, c, dt, n, i);
-    if (g.periodic) {
+    {
         for (int d = 0; d < 3; ++d) {
+            if (!g.pdim[d]) continue;
             double x = r[d];
             if (x < g.bbmin[d]) {
                 x = x + g.L[d];

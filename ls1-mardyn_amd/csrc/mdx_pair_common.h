@@ -232,11 +232,11 @@ __device__ __forceinline__ Nbr neighbour(const Geom &g, int cx, int cy, int cz, 
     for (int d = 0; d < 3; ++d) {
         b.s[d] = 0.0;
         if (c3[d] < 0) {
-            if (!g.periodic) b.ok = false;
+            if (!g.pdim[d]) b.ok = false;
             c3[d] += g.n[d];
             b.s[d] = -g.L[d];
         } else if (c3[d] >= g.n[d]) {
-            if (!g.periodic) b.ok = false;
+            if (!g.pdim[d]) b.ok = false;
             c3[d] -= g.n[d];
             b.s[d] = g.L[d];
         }

@@ -16,6 +16,7 @@ LIB_PATH = HERE / "libls1amd.so"
 
 PREC_FP64 = 0
 PREC_FP32_FP64ACC = 1
+BOUNDARY_PERIODIC, BOUNDARY_OUTFLOW, BOUNDARY_REFLECTING = 0, 1, 2
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -30,7 +31,7 @@ EXPORTS = (
     "mdx_cell_ids", "mdx_cell_geometry", "mdx_cell_pairs", "mdx_set_counting", "mdx_counters",
     "mdx_upload_device", "mdx_set_timing", "mdx_last_kernel_ms", "mdx_set_charges",
     "mdx_adjoint", "mdx_lj_tables", "mdx_set_replicas", "mdx_set_pair_config", "mdx_autotune",
-    "mdx_set_thermostat", "mdx_thermostat_state", "mdx_set_rmm",
+    "mdx_set_thermostat", "mdx_thermostat_state", "mdx_set_rmm", "mdx_set_boundaries",
 )
 
 _lib = None
@@ -81,6 +82,7 @@ def load():
     L.mdx_set_thermostat.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int]
     L.mdx_thermostat_state.argtypes = [vp, _dp]
     L.mdx_set_rmm.argtypes = [vp, C.c_double]
+    L.mdx_set_boundaries.argtypes = [vp, _ip]
     L.mdx_adjoint.argtypes = [vp, C.c_int64] + [vp] * 8
     _lib = L
     return L
@@ -180,6 +182,11 @@ class Context:
         self._chk(self.L.mdx_set_thermostat(self.h, nrep, tp, int(bool(lrc)),
                                             int(bool(explosion_heuristics))),
                   "mdx_set_thermostat")
+
+    def set_boundaries(self, types):
+        """Per-dimension boundary types: BOUNDARY_PERIODIC or BOUNDARY_REFLECTING."""
+        t = np.ascontiguousarray(types, np.int32)
+        self._chk(self.L.mdx_set_boundaries(self.h, t), "mdx_set_boundaries")
 
     def set_rmm(self, dt):
         """Reduced-memory one-centre mode (half-step velocities, kick in the

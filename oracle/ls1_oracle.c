@@ -59,7 +59,9 @@ typedef struct {
     double *eps24, *sig2, *shift6; /* nlj_total x nlj_total */
     double epsRF, rc, rcLJ, epsRFInvrc3;
     double bbmin[3], bbmax[3];
-    int periodic;
+    int periodic;   /* any dimension periodic */
+    int pdim[3];    /* periodic per dimension */
+    int wall[3];    /* reflecting walls per dimension */
     /* LinkedCells geometry */
     int boxcells[3], cpd[3];
     double cell_len[3], cell_rlen[3], halo_min[3], halo_max[3];
@@ -251,6 +253,10 @@ void *ox_create(int ncomp, const double *desc, int desc_len, const double *mix,
         x->bbmax[d] = box[d];
     }
     x->periodic = periodic;
+    for (int d = 0; d < 3; ++d) {
+        x->pdim[d] = periodic ? 1 : 0;
+        x->wall[d] = 0;
+    }
     build_lj_tables(x, mix);
     build_geometry(x, cell_mode);
     return x;
@@ -421,6 +427,7 @@ int ox_c08_pairs(void *h, int *out, int max_pairs) {
 void ox_wrap(void *h, int n, double *r) {
     const ox_ctx *x = (const ox_ctx *)h;
     for (int d = 0; d < 3; ++d) {
+        if (!x->pdim[d]) continue;
         double L = x->bbmax[d] - x->bbmin[d];
         for (int i = 0; i < n; ++i) {
             double v = r[3 * i + d];
@@ -864,6 +871,9 @@ static void trav_cb(void *u, long c1, long c2) {
 static int make_halo(const ox_ctx *x, ox_mol **pm, int n, int *cap) {
     int total = n;
     for (int d = 0; d < 3; ++d) {
+        /* non-periodic dimensions keep no halo copies
+         * (BoundaryHandler::removeNonPeriodicHalos, BoundaryHandler.cpp:136-170) */
+        if (!x->pdim[d]) continue;
         double L = x->bbmax[d] - x->bbmin[d];
         for (int dir = 0; dir < 2; ++dir) {
             double shift = dir == 0 ? L : -L;
@@ -964,6 +974,7 @@ int ox_forces_brute(void *h, int n, const double *r, double *q, const int *cid, 
             double s[3] = {0, 0, 0};
             if (x->periodic)
                 for (int d = 0; d < 3; ++d) {
+                    if (!x->pdim[d]) continue;
                     double L = x->bbmax[d] - x->bbmin[d];
                     double dd = mols[i].r[d] - mols[j].r[d];
                     if (dd > 0.5 * L) s[d] = L;
@@ -1066,6 +1077,61 @@ void ox_leapfrog_post(void *h, int n, double dt, double *v, double *L, const dou
     sums[3] = rot;
 }
 
+/* <boundaries> per dimension (Simulation.cpp:429-444): 0 periodic, 2
+ * reflecting (BoundaryUtils::BoundaryType order PERIODIC, OUTFLOW,
+ * REFLECTING); outflow (1) is not restated. */
+int ox_set_boundaries(void *h, const int *types) {
+    ox_ctx *x = (ox_ctx *)h;
+    int any = 0;
+    for (int d = 0; d < 3; ++d) {
+        if (types[d] != 0 && types[d] != 2) return -1;
+        x->pdim[d] = types[d] == 0;
+        x->wall[d] = types[d] == 2;
+        any |= x->pdim[d];
+    }
+    x->periodic = any;
+    return 0;
+}
+
+/* BoundaryHandler::processGlobalWallLeavingParticles for reflecting walls
+ * (BoundaryHandler.cpp:82-134), walls in the order +x, -x, +y, -y, +z, -z:
+ * a molecule inside the wall's slab of width rc (RegionUtils::
+ * getInnerRegionSlab, RegionUtils.cpp:11-45; inBox l <= r < u) whose
+ * predicted position r + dt (v + dt/2m F) reaches the wall
+ * (RegionUtils::isMoleculeLeaving, RegionUtils.cpp:47-56) gets
+ * v = -v - 2 (dt/2m) F. */
+void ox_reflect(void *h, int n, double dt, const double *r, double *v, const double *F,
+                const int *cid) {
+    const ox_ctx *x = (const ox_ctx *)h;
+    for (int wall = 0; wall < 6; ++wall) {
+        const int d = wall / 2, pos = (wall % 2) == 0;
+        if (!x->wall[d]) continue;
+        double lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = x->bbmin[k];
+            hi[k] = x->bbmax[k];
+        }
+        if (pos)
+            lo[d] = fmax(x->bbmax[d] - x->rc, x->bbmin[d]);
+        else
+            hi[d] = fmin(x->bbmin[d] + x->rc, x->bbmax[d]);
+        for (int i = 0; i < n; ++i) {
+            const double *ri = r + 3 * i;
+            int in = 1;
+            for (int k = 0; k < 3; ++k) in &= (ri[k] >= lo[k] && ri[k] < hi[k]);
+            if (!in) continue;
+            const double half = .5 * dt;
+            const double hm = half / x->comp[cid[i]].m;
+            const double adj = hm * F[3 * i + d];
+            const double np = ri[d] + (dt * (v[3 * i + d] + adj));
+            if ((pos && np >= x->bbmax[d]) || (!pos && np <= x->bbmin[d]))
+                v[3 * i + d] = -v[3 * i + d] - adj - adj;
+        }
+    }
+}
+
+static int any_wall(const ox_ctx *x) { return x->wall[0] | x->wall[1] | x->wall[2]; }
+
 /* Simulation::simulateOneTimestep step order (Simulation.cpp:1076-1307)
  * without thermostat, long-range correction or plugins: upd_preF, periodic
  * wrap, re-bin + halo + SoA, traversal, calcFM, upd_postF. F/M must hold the
@@ -1075,6 +1141,7 @@ int ox_simulate(void *h, int n, int nsteps, double dt, double *r, double *v, dou
                 double *L, const int *cid, double *F, double *M, double *Vi, double *per_step) {
     double mac[4], sums[4];
     for (int s = 0; s < nsteps; ++s) {
+        if (any_wall((ox_ctx *)h)) ox_reflect(h, n, dt, r, v, F, cid);
         ox_leapfrog_pre(h, n, dt, r, v, q, L, F, M, cid);
         if (((ox_ctx *)h)->periodic) ox_wrap(h, n, r);
         if (ox_forces(h, n, r, q, cid, F, M, Vi, mac, NULL)) return -1;
@@ -1284,6 +1351,7 @@ int ox_simulate_nvt(void *h, int n, int nsteps, double dt, double *r, double *v,
                     const double *lrc, double Ttarget, int heuristics, double *per_step) {
     double mac[4], sums[4], tp[4];
     for (int s = 0; s < nsteps; ++s) {
+        if (any_wall((ox_ctx *)h)) ox_reflect(h, n, dt, r, v, F, cid);
         ox_leapfrog_pre(h, n, dt, r, v, q, L, F, M, cid);
         if (((ox_ctx *)h)->periodic) ox_wrap(h, n, r);
         if (ox_forces(h, n, r, q, cid, F, M, Vi, mac, NULL)) return -1;
@@ -1424,6 +1492,7 @@ int ox_adjoint(void *h, int n, const double *r, double *q, const int *cid, const
             double sft[3] = {0, 0, 0};
             if (x->periodic)
                 for (int d = 0; d < 3; ++d) {
+                    if (!x->pdim[d]) continue;
                     double L = x->bbmax[d] - x->bbmin[d], dd = mols[i].r[d] - mols[j].r[d];
                     if (dd > 0.5 * L) sft[d] = L;
                     else if (dd < -0.5 * L) sft[d] = -L;

@@ -64,6 +64,12 @@ int ox_simulate(void *h, int n, int nsteps, double dt, double *r, double *v,
                 double *q, double *L, const int *cid, double *F, double *M,
                 double *Vi, double *per_step);
 
+/* per-dimension boundaries (0 periodic, 2 reflecting) and the reflecting
+ * walls' velocity reversal; see ls1_oracle.c */
+int ox_set_boundaries(void *h, const int *types);
+void ox_reflect(void *h, int n, double dt, const double *r, double *v, const double *F,
+                const int *cid);
+
 /* global values, homogeneous long-range correction, velocity-scaling
  * thermostat; see ls1_oracle.c */
 int ox_lrc_homogeneous(void *h, const double *nmol, double volume, double *out);

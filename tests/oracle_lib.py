@@ -69,6 +69,8 @@ class Oracle:
             L.ox_thermostat_params.argtypes = [_dp, C.c_double, C.c_int, _dp]
             L.ox_velocity_scaling.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip,
                                               _dp, C.c_double]
+            L.ox_set_boundaries.argtypes = [C.c_void_p, _ip]
+            L.ox_reflect.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp, _dp, _dp, _ip]
             L.ox_simulate_rmm.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp,
                                           _ip, _dp]
             L.ox_simulate_nvt.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp,
@@ -251,6 +253,17 @@ class Oracle:
             raise RuntimeError("oracle simulate failed")
         return dict(r=r, v=v, q=q, L=L, F=F, M=M, Vi=Vi, per_step=per.reshape(-1, 9)[:nsteps],
                     lrc=terms)
+
+    def set_boundaries(self, types):
+        """Per dimension: 0 periodic, 2 reflecting."""
+        if self.lib().ox_set_boundaries(self.h, np.ascontiguousarray(types, np.int32)):
+            raise ValueError("unsupported boundary type")
+
+    def reflect(self, dt, r, v, F, cid):
+        v = _f64(v).copy()
+        self.lib().ox_reflect(self.h, len(v), float(dt), _f64(r), v, _f64(F),
+                              np.ascontiguousarray(cid, np.int32))
+        return v
 
     def simulate_rmm(self, nsteps, dt, r, v, q, cid):
         """Reduced-memory mode: initial half kick, then nsteps; returns the
