@@ -42,17 +42,6 @@ template <typename R> struct PO {
     R u, rf;
 };
 
-// _loopBodyLJ (VectorizedCellProcessor.cpp:171-224); u = 6 * U_LJ + shift6
-template <typename R>
-__device__ __forceinline__ void body_lj(V3<R> d, R eps24, R sig2, R shift6, PO<R> &o) {
-    R r2inv = Math<R>::rcp(dot3(d, d));
-    R lj2 = sig2 * r2inv, lj6 = lj2 * lj2 * lj2, lj12 = lj6 * lj6;
-    R lj12m6 = lj12 - lj6;
-    R scale = eps24 * r2inv * (lj12 + lj12m6);
-    o.f = {d.x * scale, d.y * scale, d.z * scale};
-    o.u = eps24 * lj12m6 + shift6;
-}
-
 // _loopBodyCharge (VectorizedCellProcessor.cpp:227-277)
 template <typename R> __device__ __forceinline__ void body_cc(V3<R> d, R q1, R q2, PO<R> &o) {
     R r2 = dot3(d, d);

@@ -26,7 +26,44 @@ namespace mdx {
 
 template <typename R> __device__ __forceinline__ V3<R> v3(const Q4<R> &q) { return {q.x, q.y, q.z}; }
 
-// One site pair seen from site i (tile entry s): force on i, virial
+// LJ pair seen from site i, evaluated on every lane of the wave: a lane
+// whose entry is not accepted (ok false) runs the body at a huge r^2 (FP32:
+// +inf; FP64: 1e300, which keeps the Newton steps of 1/r^2 finite), where
+// force and virial come out exactly 0, and adds no energy; u = 6 U_LJ +
+// shift6 (_loopBodyLJ, VectorizedCellProcessor.cpp:171-224). FP32 form: pair terms
+// in FP32 up to the radial factor, the force and virial products formed in
+// FP64 from exact conversions.
+template <typename R, bool MOLSITE, bool MULTI>
+__device__ __forceinline__ void eval_lj(const PairArgs &a, const ISite<R> &I, const Q4<R> &pj,
+                                        const Q4<R> &mj, bool ok, R eps24, R sig2, R shift6,
+                                        Acc &acc) {
+    const V3<R> d = {I.pos.x - pj.x, I.pos.y - pj.y, I.pos.z - pj.z};
+    V3<R> dm = d;
+    if constexpr (!MOLSITE) dm = {I.mol.x - mj.x, I.mol.y - mj.y, I.mol.z - mj.z};
+    if constexpr (MULTI) {
+        const int k = I.ljid * a.pp.nlj + (int)pj.w;
+        if constexpr (sizeof(R) == 8) {
+            eps24 = a.pp.eps24[k]; sig2 = a.pp.sig2[k]; shift6 = a.pp.shift6[k];
+        } else {
+            eps24 = a.pp.feps24[k]; sig2 = a.pp.fsig2[k]; shift6 = a.pp.fshift6[k];
+        }
+    }
+    const R far = sizeof(R) == 4 ? R(__builtin_inff()) : R(1e300);
+    const R r2inv = Math<R>::rcp(ok ? dot3(d, d) : far);
+    const R lj2 = sig2 * r2inv, lj6 = lj2 * lj2 * lj2, lj12 = lj6 * lj6;
+    const R lj12m6 = lj12 - lj6;
+    const double s = (double)(eps24 * r2inv * (lj12 + lj12m6));
+    const double fx = (double)d.x * s, fy = (double)d.y * s, fz = (double)d.z * s;
+    acc.f[0] += fx;
+    acc.f[1] += fy;
+    acc.f[2] += fz;
+    acc.v[0] += (double)dm.x * fx;
+    acc.v[1] += (double)dm.y * fy;
+    acc.v[2] += (double)dm.z * fz;
+    acc.u += (double)(ok ? eps24 * lj12m6 + shift6 : R(0));
+}
+
+// One electrostatic site pair seen from site i (tile entry s): force on i, virial
 // (m_i - m_j) * f_i, torque on i; roles swapped where the reference swaps.
 // pj = T.p[s]; mj = T.m[s] where the tile keeps molecule positions (read
 // ahead by the caller).
@@ -42,51 +79,40 @@ __device__ __forceinline__ void eval_pair(const PairArgs &a, const ISite<R> &I,
     o.rf = R(0);
     R sign = R(1);
     V3<R> tq = {R(0), R(0), R(0)};
-    if constexpr (TI == T_LJ) {
-        if constexpr (MULTI) {
-            int k = I.ljid * a.pp.nlj + (int)pj.w;
-            if constexpr (sizeof(R) == 8) {
-                eps24 = a.pp.eps24[k]; sig2 = a.pp.sig2[k]; shift6 = a.pp.shift6[k];
-            } else {
-                eps24 = a.pp.feps24[k]; sig2 = a.pp.fsig2[k]; shift6 = a.pp.fshift6[k];
-            }
-        }
-        body_lj<R>(d, eps24, sig2, shift6, o);
-    } else {
-        V3<R> ej = {R(0), R(0), R(0)};
-        if constexpr (TJ >= T_D) ej = v3(T.e[s]);
-        const R qj = pj.w;
-        if constexpr (TI == T_C && TJ == T_C) {
-            body_cc<R>(d, I.par, qj, o);
-        } else if constexpr (TI == T_C && TJ == T_D) {
-            body_cd<R>(d, I.par, ej, qj, o);
-        } else if constexpr (TI == T_C && TJ == T_Q) {
-            body_cq<R>(d, I.par, ej, qj, o);
-        } else if constexpr (TI == T_D && TJ == T_C) {
-            V3<R> nd = {-d.x, -d.y, -d.z};
-            body_cd<R>(nd, qj, I.e, I.par, o);
-            sign = R(-1);
-            tq = o.M2;
-        } else if constexpr (TI == T_D && TJ == T_D) {
-            body_dd<R>(d, I.e, I.par, ej, qj, (R)a.epsRFInvrc3, o);
-            tq = o.M1;
-        } else if constexpr (TI == T_D && TJ == T_Q) {
-            body_dq<R>(d, I.e, I.par, ej, qj, o);
-            tq = o.M1;
-        } else if constexpr (TI == T_Q && TJ == T_C) {
-            V3<R> nd = {-d.x, -d.y, -d.z};
-            body_cq<R>(nd, qj, I.e, I.par, o);
-            sign = R(-1);
-            tq = o.M2;
-        } else if constexpr (TI == T_Q && TJ == T_D) {
-            V3<R> nd = {-d.x, -d.y, -d.z};
-            body_dq<R>(nd, ej, qj, I.e, I.par, o);
-            sign = R(-1);
-            tq = o.M2;
-        } else if constexpr (TI == T_Q && TJ == T_Q) {
-            body_qq<R>(d, I.e, I.par, ej, qj, o);
-            tq = o.M1;
-        }
+    static_assert(TI != T_LJ, "LJ pairs go through eval_lj");
+    V3<R> ej = {R(0), R(0), R(0)};
+    if constexpr (TJ >= T_D) ej = v3(T.e[s]);
+    const R qj = pj.w;
+    if constexpr (TI == T_C && TJ == T_C) {
+        body_cc<R>(d, I.par, qj, o);
+    } else if constexpr (TI == T_C && TJ == T_D) {
+        body_cd<R>(d, I.par, ej, qj, o);
+    } else if constexpr (TI == T_C && TJ == T_Q) {
+        body_cq<R>(d, I.par, ej, qj, o);
+    } else if constexpr (TI == T_D && TJ == T_C) {
+        V3<R> nd = {-d.x, -d.y, -d.z};
+        body_cd<R>(nd, qj, I.e, I.par, o);
+        sign = R(-1);
+        tq = o.M2;
+    } else if constexpr (TI == T_D && TJ == T_D) {
+        body_dd<R>(d, I.e, I.par, ej, qj, (R)a.epsRFInvrc3, o);
+        tq = o.M1;
+    } else if constexpr (TI == T_D && TJ == T_Q) {
+        body_dq<R>(d, I.e, I.par, ej, qj, o);
+        tq = o.M1;
+    } else if constexpr (TI == T_Q && TJ == T_C) {
+        V3<R> nd = {-d.x, -d.y, -d.z};
+        body_cq<R>(nd, qj, I.e, I.par, o);
+        sign = R(-1);
+        tq = o.M2;
+    } else if constexpr (TI == T_Q && TJ == T_D) {
+        V3<R> nd = {-d.x, -d.y, -d.z};
+        body_dq<R>(nd, ej, qj, I.e, I.par, o);
+        sign = R(-1);
+        tq = o.M2;
+    } else if constexpr (TI == T_Q && TJ == T_Q) {
+        body_qq<R>(d, I.e, I.par, ej, qj, o);
+        tq = o.M1;
     }
     V3<R> f = {sign * o.f.x, sign * o.f.y, sign * o.f.z};
     acc.f[0] += (double)f.x;
@@ -237,58 +263,81 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         // is 0 (the molecule itself, SingleCellPolicy_) or, in FP32, lies in
         // the band [lo, hi] around rc^2, where the decision is re-made in FP64
         // from the unrounded positions so the pair set equals the FP64 one.
+        auto recheck = [&](int s) -> bool {  // the FP64 decision for tile entry s
+            const int idx = base + s;
+            const int k = tile_k(T.pre, idx);
+            const int j = T.j0[k] + (idx - T.pre[k]);
+            const SiteArrays &S = a.s[TI];
+            const double ex = S.mx[I.idx] - (J.mx[j] + T.shift[k][0]);
+            const double ey = S.my[I.idx] - (J.my[j] + T.shift[k][1]);
+            const double ez = S.mz[I.idx] - (J.mz[j] + T.shift[k][2]);
+            const double r2d = ex * ex + ey * ey + ez * ez;
+            return r2d < rc2 && r2d != 0.0;
+        };
         auto accept = [&](int s, const Q4<R> &q) -> bool {
             const V3<R> mi = C::MOL ? I.mol : I.pos;
             const R dx = mi.x - q.x, dy = mi.y - q.y, dz = mi.z - q.z;
             const R r2 = dx * dx + dy * dy + dz * dz;
             if (r2 == R(0)) return false;
             if constexpr (sizeof(R) == 4) {
-                if (r2 >= lo) {
-                    const int idx = base + s;
-                    const int k = tile_k(T.pre, idx);
-                    const int j = T.j0[k] + (idx - T.pre[k]);
-                    const SiteArrays &S = a.s[TI];
-                    const double ex = S.mx[I.idx] - (J.mx[j] + T.shift[k][0]);
-                    const double ey = S.my[I.idx] - (J.my[j] + T.shift[k][1]);
-                    const double ez = S.mz[I.idx] - (J.mz[j] + T.shift[k][2]);
-                    const double r2d = ex * ex + ey * ey + ez * ez;
-                    return r2d < rc2 && r2d != 0.0;
-                }
+                if (r2 >= lo) return recheck(s);
             }
             return true;
         };
-        // the next entry's index and cutoff record are read one iteration
-        // ahead, so their LDS latency overlaps the current body
         auto drain = [&]() {
-            int sn = 0;
-            Q4<R> qn{};
-            if (cnt > 0) {
-                sn = (int)T.lst[0][lane];
-                qn = C::MOL ? T.m[sn] : T.p[sn];
-            }
-            for (int e = 0;; ++e) {
-                const bool act = e < cnt;
-                if (!__any(act)) break;
-                const int sj = sn;
-                const Q4<R> qj = qn;
-                if (e + 1 < cnt) {
-                    sn = (int)T.lst[e + 1][lane];
+            if constexpr (TI == T_LJ) {
+                // LJ: every lane runs every entry up to the longest list
+                // (exhausted lanes on entry 0 with the pair masked off), so
+                // the loop has no divergent branch but the FP32 form's rare
+                // re-check
+                for (int e = 0; __any(e < cnt); ++e) {
+                    const bool act = e < cnt;
+                    const int sj = act ? (int)T.lst[e][lane] : 0;
+                    const Q4<R> qj = C::MOL ? T.m[sj] : T.p[sj];
+                    const V3<R> mi = C::MOL ? I.mol : I.pos;
+                    const R dx = mi.x - qj.x, dy = mi.y - qj.y, dz = mi.z - qj.z;
+                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    bool ok = act && r2 != R(0);
+                    if constexpr (sizeof(R) == 4) {
+                        if (ok && r2 >= lo) ok = recheck(sj);
+                    }
+                    const Q4<R> pj = C::MOL ? T.p[sj] : qj;
+                    eval_lj<R, MOLSITE, MULTI>(a, I, pj, qj, ok, eps24, sig2, shift6, acc);
+                    if (COUNT) nhit += ok;
+                }
+            } else {
+                // the next entry's index and cutoff record are read one
+                // iteration ahead, so their LDS latency overlaps the current body
+                int sn = 0;
+                Q4<R> qn{};
+                if (cnt > 0) {
+                    sn = (int)T.lst[0][lane];
                     qn = C::MOL ? T.m[sn] : T.p[sn];
                 }
-                if (act && accept(sj, qj)) {
-                    if constexpr (heads) {
-                        const int ns = (int)qj.w;  // the molecule's sites sj .. sj + ns - 1
-                        for (int t = 0; t < ns; ++t) {
-                            const Q4<R> pj = T.p[sj + t];
-                            eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj + t, pj, qj, eps24,
+                for (int e = 0;; ++e) {
+                    const bool act = e < cnt;
+                    if (!__any(act)) break;
+                    const int sj = sn;
+                    const Q4<R> qj = qn;
+                    if (e + 1 < cnt) {
+                        sn = (int)T.lst[e + 1][lane];
+                        qn = C::MOL ? T.m[sn] : T.p[sn];
+                    }
+                    if (act && accept(sj, qj)) {
+                        if constexpr (heads) {
+                            const int ns = (int)qj.w;  // the molecule's sites sj .. sj + ns - 1
+                            for (int t = 0; t < ns; ++t) {
+                                const Q4<R> pj = T.p[sj + t];
+                                eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj + t, pj, qj,
+                                                                     eps24, sig2, shift6, acc);
+                            }
+                            if (COUNT) nhit += ns;
+                        } else {
+                            const Q4<R> pj = C::MOL ? T.p[sj] : qj;
+                            eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj, pj, qj, eps24,
                                                                  sig2, shift6, acc);
+                            if (COUNT) ++nhit;
                         }
-                        if (COUNT) nhit += ns;
-                    } else {
-                        const Q4<R> pj = C::MOL ? T.p[sj] : qj;
-                        eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj, pj, qj, eps24, sig2,
-                                                             shift6, acc);
-                        if (COUNT) ++nhit;
                     }
                 }
             }
