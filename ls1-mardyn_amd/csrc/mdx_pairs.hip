@@ -411,6 +411,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
 }
 
 __device__ __forceinline__ double shfl_xor_d(double v, int o) { return __shfl_xor(v, o, 64); }
+__device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
 
 // All i sites of one cell (global index) with their partners; writes the
 // cell's macroscopic sums to macro_part[4 * cell].
@@ -437,14 +438,14 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             }
         }
         for (int chunk = i_begin; chunk < i_end; chunk += 64) {
-            // lane = ph * Ipad + il with Ipad = pow2ceil(nI): P = 64 / Ipad
-            // phases of the chunk's nI sites split the partner candidates
+            // lane = ph * nI + il: P = floor(64 / nI) phases of the chunk's
+            // nI sites split the partner candidates; the 64 - P * nI lanes
+            // left over idle (phase 0, never a hit)
             const int nI = min(64, i_end - chunk);
-            int lg = 0;
-            while ((1 << lg) < nI) ++lg;
-            const int Ipad = 1 << lg, P = 64 >> lg;
-            const int il = lane & (Ipad - 1), ph = lane >> lg;
-            const bool valid = il < nI;
+            const int P = 64 / nI;
+            const int ph0 = lane / nI, il = lane - ph0 * nI;
+            const bool valid = ph0 < P;
+            const int ph = valid ? ph0 : 0;
             const int i = chunk + (valid ? il : 0);
             ISite<R> I;
             I.pos = SiteView<R>::pos(S, i);
@@ -476,15 +477,24 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
                                                                     lane, T, acc, eps24, sig2,
                                                                     shift6, ntest);
             }
-            // fold the phases of each site (butterfly over the phase bits; the
-            // virial sums below take the per-lane parts before the fold)
+            // fold the phases of each site into its phase-0 lane (tree over
+            // the phases: after the step of width w, lane ph holds phases
+            // ph .. ph + 2w - 1; the virial sums below take the per-lane
+            // parts before the fold)
             wmac[1] += acc.v[0] + acc.v[1] + acc.v[2];  // virial = trace of the site tensors
-            for (int o = Ipad; o < 64; o <<= 1)
+            for (int w = 1; w < P; w <<= 1) {
+                const bool take = valid && ph + w < P;
+                const int src = take ? lane + w * nI : lane;
                 for (int k = 0; k < 3; ++k) {
-                    acc.f[k] += shfl_xor_d(acc.f[k], o);
-                    acc.v[k] += shfl_xor_d(acc.v[k], o);
-                    if constexpr (TI >= T_D) acc.m[k] += shfl_xor_d(acc.m[k], o);
+                    const double f = shfl_d(acc.f[k], src), v = shfl_d(acc.v[k], src);
+                    acc.f[k] += take ? f : 0.0;
+                    acc.v[k] += take ? v : 0.0;
+                    if constexpr (TI >= T_D) {
+                        const double m = shfl_d(acc.m[k], src);
+                        acc.m[k] += take ? m : 0.0;
+                    }
                 }
+            }
             if (valid && ph == 0 && a.kvx) {
                 // VCP1CLJRMM (VCP1CLJRMM.cpp:300-362): the site's force, times
                 // dt/m, goes straight into its molecule's velocity
