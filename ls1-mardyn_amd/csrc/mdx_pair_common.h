@@ -286,8 +286,16 @@ template <int TI, typename R, bool MOLSITE, bool MULTI> struct TileCfg {
     static constexpr bool ORI = TI != T_LJ;  // partners may be dipoles / quadrupoles
     static constexpr bool MOL = !MOLSITE;    // molecule positions differ from sites
     static constexpr bool LJID = TI == T_LJ && MULTI;  // several LJ centre types
-    // sized so a one-centre LJ tile fits 4 wavefronts per SIMD
-    static constexpr int CAP = TI == T_LJ ? (sizeof(R) == 4 ? 352 : 256) : 256;
+    // Sites per LDS chunk. The kernels are bound by VALU issue, so the tile
+    // is sized for occupancy first (160 KB of LDS per CU shared by the
+    // wavefronts of its 4 SIMDs, next to the register limit of each
+    // instantiation) and chunk count second; values measured best on the
+    // 4 M-molecule LJ slab (one-centre) and the 1 M-molecule multi-site
+    // mixture (2CLJQ + water).
+    static constexpr int CAP =
+        TI == T_LJ ? (MOLSITE ? (sizeof(R) == 4 ? 352 : 224) : (sizeof(R) == 4 ? 192 : 128))
+        : TI == T_C ? (sizeof(R) == 4 ? 112 : 96)
+                    : (sizeof(R) == 4 ? 192 : 160);
 };
 
 template <int TI, typename R, bool MOLSITE, bool MULTI> struct Tile {
