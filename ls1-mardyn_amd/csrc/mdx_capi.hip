@@ -40,6 +40,7 @@ __global__ void k_calc_fm(int, MolArrays, const Component *, SiteArrays, SiteArr
 __global__ void k_velocity_scale(MolArrays, double *, double *, const Component *, Reps,
                                  const double *, int);
 __global__ void k_scatter_orig(int, const int *, const double *, double *, int, int);
+__global__ void k_normalize_q(int, MolArrays, long long);
 __global__ void k_finalize_macro(const double *, double *, double *, const int *, int,
                                  const double *);
 __global__ void k_finalize_lf(const double *, double *, const int *, int, double *, int);
@@ -72,6 +73,11 @@ struct mdx_context {
     // is molecule i, the gather builds the site cache, the pair kernel writes
     // the molecule force and virial (no calcFM pass)
     bool point_lj = false;
+    // point_lj: q and L never change (no torque, invI = 0, beta_rot = 1) and
+    // stay in upload order, shared by both state buffers; the per-update
+    // normalisation of q (setupSoACache) is applied when q is read
+    // (q_renorm = container updates since the last application)
+    int64_t q_renorm = 0;
     // NVT tail of the step (Domain::calculateGlobalValues, Homogeneous LRC,
     // velocity-scaling thermostat), off by default (the NVE path)
     bool lrc = false, thermostat = false;
@@ -556,10 +562,19 @@ Reps reps(const mdx_context *c) {
     return r;
 }
 
-int alloc_mol(mdx_context *c, MolArrays &m, int n) {
+// share: take q and L from this buffer instead of allocating them
+int alloc_mol(mdx_context *c, MolArrays &m, int n, const MolArrays *share = nullptr) {
     double **dp[] = {&m.rx, &m.ry, &m.rz, &m.vx, &m.vy, &m.vz, &m.qw,
                      &m.qx, &m.qy, &m.qz, &m.Lx, &m.Ly, &m.Lz};
-    for (double **p : dp) TRY(dalloc(c, p, n, true));
+    for (int k = 0; k < 13; ++k) {
+        if (share && k >= 6) {
+            double *const src[] = {share->qw, share->qx, share->qy, share->qz,
+                                   share->Lx, share->Ly, share->Lz};
+            *dp[k] = src[k - 6];
+        } else {
+            TRY(dalloc(c, dp[k], n, true));
+        }
+    }
     TRY(dalloc(c, &m.cid, n, true));
     TRY(dalloc(c, &m.orig, n, true));
     return MDX_OK;
@@ -688,6 +703,7 @@ SiteArrays site_view(const mdx_context *c, int t) {
 // leapfrog_pre already applied) and SoA cache build (setupSoACache).
 int launch_update(mdx_context *c) {
     const int n = c->n;
+    if (c->point_lj) ++c->q_renorm;
     MolArrays &src = c->mol[c->cur], &dst = c->mol[1 - c->cur];
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(c->ncell + 1)), dim3(kBlock), 0, c->stream, c->count,
                        c->ncell + 1, 0);
@@ -872,7 +888,9 @@ int state_setup(mdx_context *c, int64_t n64, const int32_t *cid_host) {
     }
     for (int t = 0; t < kTypes; ++t)
         if (c->nsites[t] >= (1 << 27)) return fail(c, MDX_ERR_CAPACITY, "too many sites");
-    for (auto &m : c->mol) TRY(alloc_mol(c, m, n));
+    TRY(alloc_mol(c, c->mol[0], n));
+    TRY(alloc_mol(c, c->mol[1], n, c->point_lj ? &c->mol[0] : nullptr));
+    c->q_renorm = 0;
     TRY(dalloc(c, &c->F, 3 * (size_t)n, true));
     TRY(dalloc(c, &c->M, 3 * (size_t)n, true));
     TRY(dalloc(c, &c->Vi, 3 * (size_t)n, true));
@@ -1345,6 +1363,18 @@ int mdx_download(mdx_context *c, int64_t n, double *r, double *v, double *q, dou
     const int nn = c->n;
     MolArrays &m = c->mol[c->cur];
     double *tmp = c->dl_tmp;
+    if (c->point_lj && c->q_renorm > 0) {  // the pending normalisations of q
+        hipLaunchKernelGGL(k_normalize_q, dim3(nblk(nn)), dim3(kBlock), 0, c->stream, nn, m,
+                           (long long)c->q_renorm);
+        HIPCHK(c, hipGetLastError());
+        c->q_renorm = 0;
+    }
+    // upload-order arrays (point_lj: q, L) are copied as they are
+    auto get_plain = [&](const double *a, double *out) -> int {
+        HIPCHK(c, hipMemcpyAsync(out, a, (size_t)nn * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return MDX_OK;
+    };
     auto get = [&](const double *a, int width, int stride_in, double *out) -> int {
         if (!out) return MDX_OK;
         hipLaunchKernelGGL(k_scatter_orig, dim3(nblk(nn)), dim3(kBlock), 0, c->stream, nn, m.orig,
@@ -1367,15 +1397,21 @@ int mdx_download(mdx_context *c, int64_t n, double *r, double *v, double *q, dou
     };
     TRY(get3(m.rx, m.ry, m.rz, 3, r));
     TRY(get3(m.vx, m.vy, m.vz, 3, v));
-    TRY(get3(m.Lx, m.Ly, m.Lz, 3, L));
-    if (q) {
+    auto get_rot = [&](double *const *parts, int width, double *out) -> int {
+        if (!out) return MDX_OK;
         std::vector<double> col((size_t)nn);
-        double *parts[4] = {m.qw, m.qx, m.qy, m.qz};
-        for (int w = 0; w < 4; ++w) {
-            TRY(get(parts[w], 1, nn, col.data()));
-            for (int i = 0; i < nn; ++i) q[(size_t)i * 4 + w] = col[i];
+        for (int w = 0; w < width; ++w) {
+            if (c->point_lj)
+                TRY(get_plain(parts[w], col.data()));
+            else
+                TRY(get(parts[w], 1, nn, col.data()));
+            for (int i = 0; i < nn; ++i) out[(size_t)i * width + w] = col[i];
         }
-    }
+        return MDX_OK;
+    };
+    double *const lparts[3] = {m.Lx, m.Ly, m.Lz}, *const qparts[4] = {m.qw, m.qx, m.qy, m.qz};
+    TRY(get_rot(lparts, 3, L));
+    TRY(get_rot(qparts, 4, q));
     TRY(get(c->F, 3, nn, F));
     TRY(get(c->M, 3, nn, M));
     TRY(get(c->Vi, 3, nn, Vi));
@@ -1448,6 +1484,7 @@ int mdx_step(mdx_context *c, int nsteps, double dt, double *per_step, int use_gr
         return fail(c, MDX_ERR_ARG, "reduced-memory mode: dt differs from mdx_set_rmm's");
     HIPCHK(c, hipSetDevice(c->device));
     TRY(ensure_steps(c, nsteps));
+    const int64_t q_renorm0 = c->q_renorm;
     HIPCHK(c, hipMemsetAsync(c->step_idx, 0, sizeof(int), c->stream));
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     if (use_graph && !c->counting && !c->timing) {
@@ -1473,6 +1510,8 @@ int mdx_step(mdx_context *c, int nsteps, double dt, double *per_step, int use_gr
     } else {
         for (int s = 0; s < nsteps; ++s) TRY(launch_step(c, dt));
     }
+    // one container update per step (a captured step counted none)
+    if (c->point_lj) c->q_renorm = q_renorm0 + nsteps;
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     if (per_step && nsteps > 0)
         HIPCHK(c, hipMemcpyAsync(per_step, c->step_out,

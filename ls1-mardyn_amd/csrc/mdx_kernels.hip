@@ -349,7 +349,8 @@ __global__ __launch_bounds__(256) void k_bin_scatter(int n, const int *__restric
 
 // Stable order inside each cell (by previous position, which makes the sort
 // deterministic and keeps molecules that stay in their cell in order), then
-// the permutation of the state into cell order; q is left to the caller.
+// the permutation of the state into cell order; q and L are left to the
+// caller.
 // Returns the destination index; *src_i, *cell_of the source and the cell.
 __device__ __forceinline__ int gather_common(int p, const MolArrays &src, const MolArrays &dst,
                                              const int *__restrict__ cell,
@@ -365,7 +366,6 @@ __device__ __forceinline__ int gather_common(int p, const MolArrays &src, const 
     int d = b + r;
     dst.rx[d] = src.rx[i]; dst.ry[d] = src.ry[i]; dst.rz[d] = src.rz[i];
     dst.vx[d] = src.vx[i]; dst.vy[d] = src.vy[i]; dst.vz[d] = src.vz[i];
-    dst.Lx[d] = src.Lx[i]; dst.Ly[d] = src.Ly[i]; dst.Lz[d] = src.Lz[i];
     dst.cid[d] = src.cid[i];
     dst.orig[d] = src.orig[i];
     newcell[d] = c;
@@ -388,6 +388,7 @@ __global__ __launch_bounds__(256) void k_bin_gather(int n, MolArrays src, MolArr
     int i, c;
     const int d = gather_common(p, src, dst, cell, perm, start, newcell, ls1id, newls1id, &i, &c);
     dst.qw[d] = src.qw[i]; dst.qx[d] = src.qx[i]; dst.qy[d] = src.qy[i]; dst.qz[d] = src.qz[i];
+    dst.Lx[d] = src.Lx[i]; dst.Ly[d] = src.Ly[i]; dst.Lz[d] = src.Lz[i];
     const Component &cp = comps[src.cid[i]];
     nsite0[d] = cp.n[0];
     nsite1[d] = cp.n[1];
@@ -397,10 +398,11 @@ __global__ __launch_bounds__(256) void k_bin_gather(int n, MolArrays src, MolArr
 
 // The same for systems whose every component is one LJ centre at the
 // molecule's centre and nothing else: site j is molecule j, so the gather
-// also does the SoA cache build of k_build_sites for them: q normalised
-// (FullMolecule.cpp:719), the FP32 position relative to the cell corner and
-// the LJ lookup id. Site FP64 positions, forces and virials are the molecule
-// arrays themselves.
+// also does the SoA cache build of k_build_sites for them: the FP32 position
+// relative to the cell corner and the LJ lookup id. Site FP64 positions,
+// forces and virials are the molecule arrays themselves. q and L of such
+// molecules never change and stay in upload order (q's normalisation of
+// setupSoACache, FullMolecule.cpp:719, is applied by k_normalize_q).
 __global__ __launch_bounds__(256) void k_bin_gather_point(
     int n, MolArrays src, MolArrays dst, const int *__restrict__ cell,
     const int *__restrict__ perm, const int *__restrict__ start, int *newcell,
@@ -410,10 +412,6 @@ __global__ __launch_bounds__(256) void k_bin_gather_point(
     if (p >= n) return;
     int i, c;
     const int d = gather_common(p, src, dst, cell, perm, start, newcell, ls1id, newls1id, &i, &c);
-    double q[4] = {src.qw[i], src.qx[i], src.qy[i], src.qz[i]};
-    double qs = 1. / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-    for (int k = 0; k < 4; ++k) q[k] *= qs;
-    dst.qw[d] = q[0]; dst.qx[d] = q[1]; dst.qy[d] = q[2]; dst.qz[d] = q[3];
     const int rep = cell_replica(R, c);
     const Geom &g = R.geom[rep];
     const int local = c - R.cell_off[rep];
@@ -422,6 +420,38 @@ __global__ __launch_bounds__(256) void k_bin_gather_point(
     fy[d] = (float)(src.ry[i] - (g.bbmin[1] + cy * g.cl[1]));
     fz[d] = (float)(src.rz[i] - (g.bbmin[2] + cz * g.cl[2]));
     ljid[d] = comps[src.cid[i]].lj_base;
+}
+
+// q <- N^k(q), N = Quaternion::normalize (q scaled by 1/sqrt(|q|^2)), for
+// molecules whose q is touched by nothing else (one-centre systems, where ls1
+// normalises every molecule at each of the k container updates). N reaches a
+// fixed point or a two-cycle after a few applications; the rest of the k are
+// skipped, or reduced to their parity for a two-cycle.
+__global__ __launch_bounds__(256) void k_normalize_q(int n, MolArrays s, long long k) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]}, qp[4] = {0, 0, 0, 0};
+    for (long long t = 0; t < k; ++t) {
+        const double qs = 1. / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        double qn[4];
+        for (int c = 0; c < 4; ++c) qn[c] = q[c] * qs;
+        bool same = true, cyc = t > 0;
+        for (int c = 0; c < 4; ++c) {
+            same = same && qn[c] == q[c];
+            cyc = cyc && qn[c] == qp[c];
+        }
+        if (same) break;
+        if (cyc) {  // q <-> qn: the remaining k - t applications alternate
+            if ((k - t) & 1)
+                for (int c = 0; c < 4; ++c) q[c] = qn[c];
+            break;
+        }
+        for (int c = 0; c < 4; ++c) {
+            qp[c] = q[c];
+            q[c] = qn[c];
+        }
+    }
+    s.qw[i] = q[0]; s.qx[i] = q[1]; s.qy[i] = q[2]; s.qz[i] = q[3];
 }
 
 // ------------------------------------------------------------- scan (int)
@@ -661,9 +691,15 @@ __global__ __launch_bounds__(256) void k_velocity_scale(MolArrays s, double *F, 
     const double Ti = th[kThTarget];
     if (!(Ti > 0.0)) return;
     const Component &c = comps[s.cid[i]];
+    const double bt = th[kThBetaT], br = th[kThBetaR];
+    const bool explode = th[kThExplode] != 0.0;
     double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
-    double L[3] = {s.Lx[i], s.Ly[i], s.Lz[i]};
-    if (th[kThExplode] != 0.0) {
+    // L is rewritten only where it changes: beta_rot is exactly 1 in systems
+    // without rotational degrees of freedom, whose L stays in upload order
+    bool rot = br != 1.0 || (explode && c.rotdof > 0);
+    double L[3] = {0., 0., 0.};
+    if (rot) L[0] = s.Lx[i], L[1] = s.Ly[i], L[2] = s.Lz[i];
+    if (explode) {
         const double limit = 10.0 * Ti;
         const double ut = 0.5 * c.m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         if (ut > limit) {
@@ -692,9 +728,10 @@ __global__ __launch_bounds__(256) void k_velocity_scale(MolArrays s, double *F, 
             }
         }
     }
-    const double bt = th[kThBetaT], br = th[kThBetaR];
     s.vx[i] = v[0] * bt; s.vy[i] = v[1] * bt; s.vz[i] = v[2] * bt;
-    s.Lx[i] = L[0] * br; s.Ly[i] = L[1] * br; s.Lz[i] = L[2] * br;
+    if (rot) {
+        s.Lx[i] = L[0] * br; s.Ly[i] = L[1] * br; s.Lz[i] = L[2] * br;
+    }
 }
 
 // Segmented fixed-order sum: block r adds rows [off[r], off[r+1]) of a
