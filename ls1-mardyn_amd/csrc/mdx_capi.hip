@@ -15,6 +15,8 @@
 namespace mdx {
 __global__ void k_leapfrog_pre(MolArrays, const double *, const double *, const Component *, Reps,
                                double, int);
+__global__ void k_leapfrog_post_pre(MolArrays, const double *, const double *, const Component *,
+                                    Reps, double, int, const int *, const int *, double *);
 __global__ void k_leapfrog_post(MolArrays, const double *, const double *, const Component *,
                                 double, int, const int *, const int *, double *, int);
 __global__ void k_seg_reduce(const double *, int, const int *, double *);
@@ -131,7 +133,9 @@ struct mdx_context {
     std::vector<uint64_t> ids;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double pair_ms = 0, total_ms = 0;
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    // captured steps per kind (full step, step fused with the next one's
+    // first half, last step of a fused run) and state-buffer parity
+    hipGraphExec_t gexec[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     double graph_dt = 0;
     std::vector<void *> allocs;        // lifetime of the context
     std::vector<void *> state_allocs;  // re-made by each upload
@@ -176,11 +180,12 @@ void free_list(std::vector<void *> &v) {
 }
 
 void drop_graphs(mdx_context *c) {
-    for (auto &g : c->gexec)
-        if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
+    for (auto &gk : c->gexec)
+        for (auto &g : gk)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
 }
 
 // Component::calculateLJshift (Component.cpp:118-122)
@@ -836,14 +841,40 @@ int launch_lf_post(mdx_context *c, double dt) {
     return MDX_OK;
 }
 
-int launch_step(mdx_context *c, double dt) {
-    TRY(launch_lf_pre(c, dt));
+// Step n's second half kick and step n + 1's first half kick and drift in
+// one pass (no thermostat acts between them).
+int launch_lf_post_pre(mdx_context *c, double dt) {
+    const int R = c->nrep, nb = c->lf_nchunk;
+    hipLaunchKernelGGL(k_leapfrog_post_pre, dim3(nb), dim3(kBlock), 0, c->stream, c->mol[c->cur],
+                       c->F, c->M, c->d_comps, reps(c), dt, c->n, c->d_lf_chunk,
+                       c->d_lf_chunk_end, c->lf_part);
+    if (R == 1)
+        launch_reduce(c, c->lf_part, nb, 4, c->red + 8);
+    else
+        hipLaunchKernelGGL(k_seg_reduce, dim3(R), dim3(256), 0, c->stream, c->lf_part, 4,
+                           c->d_lf_off, c->red + 8 * R);
+    hipLaunchKernelGGL(k_finalize_lf, dim3((R + 63) / 64), dim3(64), 0, c->stream, c->red + 8 * R,
+                       c->step_out, c->step_idx, R, nullptr, c->heuristics);
+    HIPCHK(c, hipGetLastError());
+    return MDX_OK;
+}
+
+// Kinds of step launch: a whole step; a step whose first half kick was done
+// by the previous one and which does the next one's (fused); the last step
+// of such a run.
+enum { kStepFull = 0, kStepFused = 1, kStepLast = 2 };
+
+int launch_step(mdx_context *c, double dt, int kind = kStepFull) {
+    if (kind == kStepFull) TRY(launch_lf_pre(c, dt));
     TRY(launch_update(c));
     if (c->rmm_dt != 0.0)  // LeapfrogRMM: one full kick dt/m inside the force kernel
         TRY(launch_traverse(c, dt / c->model.comp[0].m));
     else
         TRY(launch_traverse(c));
-    TRY(launch_lf_post(c, dt));
+    if (kind == kStepFused)
+        TRY(launch_lf_post_pre(c, dt));
+    else
+        TRY(launch_lf_post(c, dt));
     hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, c->stream, c->step_idx);
     HIPCHK(c, hipGetLastError());
     return MDX_OK;
@@ -1487,28 +1518,38 @@ int mdx_step(mdx_context *c, int nsteps, double dt, double *per_step, int use_gr
     const int64_t q_renorm0 = c->q_renorm;
     HIPCHK(c, hipMemsetAsync(c->step_idx, 0, sizeof(int), c->stream));
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    // Consecutive steps without a thermostat between them share one pass
+    // over the molecules for step n's second and step n + 1's first half
+    // kick (NVE; the reduced-memory mode kicks inside the force kernel).
+    const bool fuse = nsteps > 1 && !c->thermostat && c->rmm_dt == 0.0;
+    auto kind_of = [&](int s) {
+        if (!fuse) return (int)kStepFull;
+        return s < nsteps - 1 ? (int)kStepFused : (int)kStepLast;
+    };
+    if (fuse) TRY(launch_lf_pre(c, dt));
     if (use_graph && !c->counting && !c->timing) {
         if (c->graph_dt != dt) drop_graphs(c);
         for (int s = 0; s < nsteps; ++s) {
-            int parity = c->cur;
-            if (!c->gexec[parity]) {
+            const int parity = c->cur, kind = kind_of(s);
+            hipGraphExec_t &gx = c->gexec[kind][parity];
+            if (!gx) {
                 hipGraph_t graph;
                 HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                int rc = launch_step(c, dt);
+                int rc = launch_step(c, dt, kind);
                 hipError_t e = hipStreamEndCapture(c->stream, &graph);
                 if (rc != MDX_OK) return rc;
                 if (e != hipSuccess)
                     return fail(c, MDX_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
-                HIPCHK(c, hipGraphInstantiate(&c->gexec[parity], graph, nullptr, nullptr, 0));
+                HIPCHK(c, hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0));
                 (void)hipGraphDestroy(graph);
                 c->graph_dt = dt;
                 c->cur = parity;  // capture recorded but did not run the step
             }
-            HIPCHK(c, hipGraphLaunch(c->gexec[parity], c->stream));
+            HIPCHK(c, hipGraphLaunch(gx, c->stream));
             c->cur = 1 - parity;
         }
     } else {
-        for (int s = 0; s < nsteps; ++s) TRY(launch_step(c, dt));
+        for (int s = 0; s < nsteps; ++s) TRY(launch_step(c, dt, kind_of(s)));
     }
     // one container update per step (a captured step counted none)
     if (c->point_lj) c->q_renorm = q_renorm0 + nsteps;

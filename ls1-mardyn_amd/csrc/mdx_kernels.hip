@@ -22,9 +22,13 @@
 namespace mdx {
 
 // ------------------------------------------------------------------ helpers
+// The quaternion helpers and the Leapfrog updates round every product and sum
+// on its own (fp contract off, as the oracle is built), so a fused pass over
+// two half steps and the separate passes give the same bits.
 
 // Quaternion::rotate (src/molecules/Quaternion.cpp:43-64)
 __device__ __forceinline__ void qrot(const double q[4], const double d[3], double o[3]) {
+#pragma clang fp contract(off)
     o[0] = o[1] = o[2] = 0.0;
     if (d[0] == 0.0 && d[1] == 0.0 && d[2] == 0.0) return;
     double ww = q[0] * q[0], xx = q[1] * q[1], yy = q[2] * q[2], zz = q[3] * q[3];
@@ -37,6 +41,7 @@ __device__ __forceinline__ void qrot(const double q[4], const double d[3], doubl
 
 // Quaternion::rotateinv (Quaternion.cpp:66-86)
 __device__ __forceinline__ void qrotinv(const double q[4], const double d[3], double o[3]) {
+#pragma clang fp contract(off)
     o[0] = o[1] = o[2] = 0.0;
     if (d[0] == 0.0 && d[1] == 0.0 && d[2] == 0.0) return;
     double ww = q[0] * q[0], xx = q[1] * q[1], yy = q[2] * q[2], zz = q[3] * q[3];
@@ -49,6 +54,7 @@ __device__ __forceinline__ void qrotinv(const double q[4], const double d[3], do
 
 // Quaternion::differentiate (Quaternion.cpp:98-103)
 __device__ __forceinline__ void qdiff(const double q[4], const double w[3], double dq[4]) {
+#pragma clang fp contract(off)
     dq[0] = .5 * (-q[1] * w[0] - q[2] * w[1] - q[3] * w[2]);
     dq[1] = .5 * (q[0] * w[0] - q[3] * w[1] + q[2] * w[2]);
     dq[2] = .5 * (q[3] * w[0] + q[0] * w[1] - q[1] * w[2]);
@@ -117,6 +123,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // setupSoACache does).
 __device__ __forceinline__ void rotate_pre(MolArrays &s, const double *__restrict__ M,
                                            const Component &c, double dt, int n, int i) {
+#pragma clang fp contract(off)
     const double dt_halve = .5 * dt;
     double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
     double L[3] = {s.Lx[i], s.Ly[i], s.Lz[i]};
@@ -139,20 +146,17 @@ __device__ __forceinline__ void rotate_pre(MolArrays &s, const double *__restric
 }
 
 // FullMolecule::upd_preF (FullMolecule.cpp:331-361), then the periodic wrap of
-// DomainDecompBase::handleDomainLeavingParticles (DomainDecompBase.cpp:199-249).
-__global__ __launch_bounds__(256) void k_leapfrog_pre(MolArrays s, const double *__restrict__ F,
-                                                      const double *__restrict__ M,
-                                                      const Component *__restrict__ comps, Reps R,
-                                                      double dt, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const Geom &g = R.geom[mol_replica(R, i)];
-    const Component &c = comps[s.cid[i]];
-    double dt_halve = .5 * dt;
-    double dtInv2m = dt_halve / c.m;
+// DomainDecompBase::handleDomainLeavingParticles (DomainDecompBase.cpp:199-249),
+// for molecule i with its velocity v and force f in registers (f null:
+// PositionCellProcessorRMM, r += dt v only). Writes r and v.
+__device__ __forceinline__ void lf_pre_mol(MolArrays &s, const double *__restrict__ M,
+                                           const Component &c, const Geom &g, double dt, int n,
+                                           int i, double v[3], const double *f) {
+#pragma clang fp contract(off)
+    const double dt_halve = .5 * dt;
+    const double dtInv2m = dt_halve / c.m;
     double r[3] = {s.rx[i], s.ry[i], s.rz[i]};
-    double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
-    if (F && (g.reflect[0] | g.reflect[1] | g.reflect[2])) {
+    if (f && (g.reflect[0] | g.reflect[1] | g.reflect[2])) {
         // BoundaryHandler::processGlobalWallLeavingParticles, reflecting walls
         // (BoundaryHandler.cpp:82-134; called before upd_preF,
         // Simulation.cpp:1114), walls in the order +x, -x, +y, -y, +z, -z: a
@@ -164,7 +168,7 @@ __global__ __launch_bounds__(256) void k_leapfrog_pre(MolArrays s, const double 
         for (int d = 0; d < 3; ++d) inside = inside && r[d] >= g.bbmin[d] && r[d] < g.bbmax[d];
         for (int d = 0; d < 3 && inside; ++d) {
             if (!g.reflect[d]) continue;
-            const double adj = dtInv2m * F[d * n + i];
+            const double adj = dtInv2m * f[d];
             // +wall: slab [max(max - rc, min), max)
             if (r[d] >= fmax(g.bbmax[d] - g.wall_rc, g.bbmin[d]) &&
                 r[d] + dt * (v[d] + adj) >= g.bbmax[d])
@@ -176,76 +180,71 @@ __global__ __launch_bounds__(256) void k_leapfrog_pre(MolArrays s, const double 
         }
     }
     for (int d = 0; d < 3; ++d) {
-        if (F) v[d] += dtInv2m * F[d * n + i];  // F null: PositionCellProcessorRMM, r only
+        if (f) v[d] += dtInv2m * f[d];
         r[d] += dt * v[d];
     }
     if (!c.point) rotate_pre(s, M, c, dt, n, i);
-    {
-        for (int d = 0; d < 3; ++d) {
-            if (!g.pdim[d]) continue;
-            double x = r[d];
-            if (x < g.bbmin[d]) {
-                x = x + g.L[d];
-                if (x >= g.bbmax[d]) x = nextafter(g.bbmax[d], -INFINITY);
-            } else if (x >= g.bbmax[d]) {
-                x = x - g.L[d];
-                if (x <= g.bbmin[d]) x = g.bbmin[d];
-            }
-            r[d] = x;
+    for (int d = 0; d < 3; ++d) {
+        if (!g.pdim[d]) continue;
+        double x = r[d];
+        if (x < g.bbmin[d]) {
+            x = x + g.L[d];
+            if (x >= g.bbmax[d]) x = nextafter(g.bbmax[d], -INFINITY);
+        } else if (x >= g.bbmax[d]) {
+            x = x - g.L[d];
+            if (x <= g.bbmin[d]) x = g.bbmin[d];
         }
+        r[d] = x;
     }
     s.rx[i] = r[0]; s.ry[i] = r[1]; s.rz[i] = r[2];
     s.vx[i] = v[0]; s.vy[i] = v[1]; s.vz[i] = v[2];
 }
 
-// FullMolecule::upd_postF (FullMolecule.cpp:363-387) with per-block FP64
-// partial sums {sum m v^2, sum I w^2, N, rotDOF} (Leapfrog.cpp:63-147).
-// Block b covers molecules [chunk[b], chunk[b] + 256) clipped to
-// chunk_end[b]: chunks never straddle two replicas.
-__global__ __launch_bounds__(256) void k_leapfrog_post(MolArrays s, const double *__restrict__ F,
-                                                       const double *__restrict__ M,
-                                                       const Component *__restrict__ comps,
-                                                       double dt, int n,
-                                                       const int *__restrict__ chunk,
-                                                       const int *__restrict__ chunk_end,
-                                                       double *partial, int rmm) {
-    __shared__ double red[4][4];
-    int i = chunk[blockIdx.x] + threadIdx.x;
-    double mv2 = 0, iw2 = 0, cnt = 0, rot = 0;
-    if (rmm && i < chunk_end[blockIdx.x]) {
-        // VelocityCellProcessorRMM (VelocityCellProcessorRMM.h:60-110): the
-        // kick happened in the force kernel; sum v^2 (no mass factor, as the
-        // reference's reduced-memory mode sums it) and N, no rotation
-        const double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
-        mv2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-        cnt = 1.0;
-    } else if (i < chunk_end[blockIdx.x]) {
-        const Component &c = comps[s.cid[i]];
-        double dt_halve = 0.5 * dt;
-        double dtInv2m = dt_halve / c.m;
-        double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
-        double v2 = 0.;
-        for (int d = 0; d < 3; ++d) {
-            v[d] += dtInv2m * F[d * n + i];
-            v2 += v[d] * v[d];
-        }
-        mv2 = c.m * v2;
-        if (!c.point) {  // point molecules: L unchanged, I w^2 = 0
-            double L[3] = {s.Lx[i], s.Ly[i], s.Lz[i]};
-            for (int d = 0; d < 3; ++d) L[d] += dt_halve * M[d * n + i];
-            double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
-            double w[3];
-            qrotinv(q, L, w);
-            for (int d = 0; d < 3; ++d) {
-                w[d] *= c.invI[d];
-                iw2 += c.I[d] * w[d] * w[d];
-            }
-            s.Lx[i] = L[0]; s.Ly[i] = L[1]; s.Lz[i] = L[2];
-        }
-        cnt = 1.0;
-        rot = c.rotdof;
-        s.vx[i] = v[0]; s.vy[i] = v[1]; s.vz[i] = v[2];
+__global__ __launch_bounds__(256) void k_leapfrog_pre(MolArrays s, const double *__restrict__ F,
+                                                      const double *__restrict__ M,
+                                                      const Component *__restrict__ comps, Reps R,
+                                                      double dt, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
+    double f[3] = {0., 0., 0.};
+    if (F) f[0] = F[i], f[1] = F[n + i], f[2] = F[2 * n + i];
+    lf_pre_mol(s, M, comps[s.cid[i]], R.geom[mol_replica(R, i)], dt, n, i, v, F ? f : nullptr);
+}
+
+// FullMolecule::upd_postF (FullMolecule.cpp:363-387) for molecule i: v (in
+// registers) and L advanced, {m v^2, I w^2} added to the sums.
+__device__ __forceinline__ void lf_post_mol(MolArrays &s, const double *__restrict__ M,
+                                            const Component &c, double dt, int n, int i,
+                                            double v[3], const double f[3], double &mv2,
+                                            double &iw2) {
+#pragma clang fp contract(off)
+    const double dt_halve = 0.5 * dt;
+    const double dtInv2m = dt_halve / c.m;
+    double v2 = 0.;
+    for (int d = 0; d < 3; ++d) {
+        v[d] += dtInv2m * f[d];
+        v2 += v[d] * v[d];
     }
+    mv2 = c.m * v2;
+    if (!c.point) {  // point molecules: L unchanged, I w^2 = 0
+        double L[3] = {s.Lx[i], s.Ly[i], s.Lz[i]};
+        for (int d = 0; d < 3; ++d) L[d] += dt_halve * M[d * n + i];
+        double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
+        double w[3];
+        qrotinv(q, L, w);
+        for (int d = 0; d < 3; ++d) {
+            w[d] *= c.invI[d];
+            iw2 += c.I[d] * w[d] * w[d];
+        }
+        s.Lx[i] = L[0]; s.Ly[i] = L[1]; s.Lz[i] = L[2];
+    }
+}
+
+// Block partial sums {sum m v^2, sum I w^2, N, rotDOF} (Leapfrog.cpp:63-147).
+__device__ __forceinline__ void lf_block_sums(double mv2, double iw2, double cnt, double rot,
+                                              double *partial) {
+    __shared__ double red[4][4];
     mv2 = wave_sum(mv2);
     iw2 = wave_sum(iw2);
     cnt = wave_sum(cnt);
@@ -260,6 +259,63 @@ __global__ __launch_bounds__(256) void k_leapfrog_post(MolArrays s, const double
         for (int k = 0; k < (int)(blockDim.x >> 6); ++k) a += red[k][threadIdx.x];
         partial[blockIdx.x * 4 + threadIdx.x] = a;
     }
+}
+
+// The second half kick of every molecule with the step's sums. Block b
+// covers molecules [chunk[b], chunk[b] + 256) clipped to chunk_end[b]: chunks
+// never straddle two replicas.
+__global__ __launch_bounds__(256) void k_leapfrog_post(MolArrays s, const double *__restrict__ F,
+                                                       const double *__restrict__ M,
+                                                       const Component *__restrict__ comps,
+                                                       double dt, int n,
+                                                       const int *__restrict__ chunk,
+                                                       const int *__restrict__ chunk_end,
+                                                       double *partial, int rmm) {
+    int i = chunk[blockIdx.x] + threadIdx.x;
+    double mv2 = 0, iw2 = 0, cnt = 0, rot = 0;
+    if (rmm && i < chunk_end[blockIdx.x]) {
+        // VelocityCellProcessorRMM (VelocityCellProcessorRMM.h:60-110): the
+        // kick happened in the force kernel; sum v^2 (no mass factor, as the
+        // reference's reduced-memory mode sums it) and N, no rotation
+        const double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
+        mv2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        cnt = 1.0;
+    } else if (i < chunk_end[blockIdx.x]) {
+        const Component &c = comps[s.cid[i]];
+        double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
+        const double f[3] = {F[i], F[n + i], F[2 * n + i]};
+        lf_post_mol(s, M, c, dt, n, i, v, f, mv2, iw2);
+        cnt = 1.0;
+        rot = c.rotdof;
+        s.vx[i] = v[0]; s.vy[i] = v[1]; s.vz[i] = v[2];
+    }
+    lf_block_sums(mv2, iw2, cnt, rot, partial);
+}
+
+// Step n's second half kick (with its sums) fused with step n + 1's first
+// half kick and drift, for consecutive steps without a thermostat between
+// them: the same operations in the same order as k_leapfrog_post followed by
+// k_leapfrog_pre, with v and F read once.
+__global__ __launch_bounds__(256) void k_leapfrog_post_pre(MolArrays s,
+                                                           const double *__restrict__ F,
+                                                           const double *__restrict__ M,
+                                                           const Component *__restrict__ comps,
+                                                           Reps R, double dt, int n,
+                                                           const int *__restrict__ chunk,
+                                                           const int *__restrict__ chunk_end,
+                                                           double *partial) {
+    int i = chunk[blockIdx.x] + threadIdx.x;
+    double mv2 = 0, iw2 = 0, cnt = 0, rot = 0;
+    if (i < chunk_end[blockIdx.x]) {
+        const Component &c = comps[s.cid[i]];
+        double v[3] = {s.vx[i], s.vy[i], s.vz[i]};
+        const double f[3] = {F[i], F[n + i], F[2 * n + i]};
+        lf_post_mol(s, M, c, dt, n, i, v, f, mv2, iw2);
+        cnt = 1.0;
+        rot = c.rotdof;
+        lf_pre_mol(s, M, c, R.geom[mol_replica(R, i)], dt, n, i, v, f);
+    }
+    lf_block_sums(mv2, iw2, cnt, rot, partial);
 }
 
 // Fixed-order reduction of per-block partials (width values per block):

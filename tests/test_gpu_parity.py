@@ -254,6 +254,28 @@ def test_graph_replay_matches_eager():
         np.testing.assert_array_equal(outs[0][1][k], outs[1][1][k])
 
 
+@pytest.mark.parametrize("sysname", ["small_lj", "small_mixture"])
+def test_fused_steps_equal_single_steps(sysname):
+    """A multi-step call does step n's second and step n+1's first half kick in
+    one pass; the result equals one-step calls bit for bit."""
+    say("fused steps", sysname)
+    s = getattr(scenarios, sysname)(seed=57)
+    outs = []
+    for split in (False, True):
+        with Context(0) as ctx:
+            ctx.setup_system(s)
+            ctx.update_container()
+            ctx.traverse()
+            if split:
+                per = np.vstack([ctx.step(1, 0.001, use_graph=True) for _ in range(7)])
+            else:
+                per = ctx.step(7, 0.001, use_graph=True)
+            outs.append((per, ctx.download()))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    for k in ("r", "v", "q", "L", "F", "M", "Vi"):
+        np.testing.assert_array_equal(outs[0][1][k], outs[1][1][k])
+
+
 @pytest.mark.parametrize("sysname,seed", [("small_lj", 61), ("small_mixture", 62)])
 def test_fp32_form_error_bound(sysname, seed):
     say("fp32 form", sysname)
