@@ -403,32 +403,49 @@ __global__ __launch_bounds__(256) void k_bin_scatter(int n, const int *__restric
     perm[start[cell[i]] + rank[i]] = i;
 }
 
-// Stable order inside each cell (by previous position, which makes the sort
-// deterministic and keeps molecules that stay in their cell in order), then
-// the permutation of the state into cell order; q and L are left to the
-// caller.
-// Returns the destination index; *src_i, *cell_of the source and the cell.
-__device__ __forceinline__ int gather_common(int p, const MolArrays &src, const MolArrays &dst,
-                                             const int *__restrict__ cell,
-                                             const int *__restrict__ perm,
-                                             const int *__restrict__ start, int *newcell,
-                                             const int *__restrict__ ls1id, int *newls1id,
-                                             int *src_i, int *cell_of) {
-    int i = perm[p];
-    int c = cell[i];
-    int b = start[c], e = start[c + 1];
+// Destination of entry p of perm (cells in order, each cell's members in
+// arrival order): its cell's start plus its stable rank in the cell, the
+// number of members with a smaller previous index (which makes the sort
+// deterministic and keeps molecules that stay in their cell in order). The
+// block's cells occupy one contiguous range of perm, staged in LDS when it
+// fits (the rank loops then read LDS); every thread of the block must call
+// this. Returns -1 for p >= n.
+constexpr int kGatherWin = 1024;
+__device__ __forceinline__ int stable_dest(int p, int n, const int *__restrict__ cell,
+                                           const int *__restrict__ perm,
+                                           const int *__restrict__ start, int *win, int *src_i,
+                                           int *cell_of) {
+    const int p0 = blockIdx.x * blockDim.x, p1 = min(p0 + (int)blockDim.x, n);
+    const int w0 = start[cell[perm[p0]]], w1 = start[cell[perm[p1 - 1]] + 1];
+    const bool staged = w1 - w0 <= kGatherWin;
+    if (staged)
+        for (int k = w0 + (int)threadIdx.x; k < w1; k += blockDim.x) win[k - w0] = perm[k];
+    __syncthreads();
+    if (p >= n) return -1;
+    const int i = perm[p];
+    const int c = cell[i];
+    const int b = start[c], e = start[c + 1];
     int r = 0;
-    for (int k = b; k < e; ++k) r += (perm[k] < i);
-    int d = b + r;
+    if (staged)
+        for (int k = b; k < e; ++k) r += win[k - w0] < i;
+    else
+        for (int k = b; k < e; ++k) r += perm[k] < i;
+    *src_i = i;
+    *cell_of = c;
+    return b + r;
+}
+
+// The permutation of the state into cell order (i -> d); q and L are left to
+// the caller.
+__device__ __forceinline__ void gather_common(int i, int d, int c, const MolArrays &src,
+                                              const MolArrays &dst, int *newcell,
+                                              const int *__restrict__ ls1id, int *newls1id) {
     dst.rx[d] = src.rx[i]; dst.ry[d] = src.ry[i]; dst.rz[d] = src.rz[i];
     dst.vx[d] = src.vx[i]; dst.vy[d] = src.vy[i]; dst.vz[d] = src.vz[i];
     dst.cid[d] = src.cid[i];
     dst.orig[d] = src.orig[i];
     newcell[d] = c;
     newls1id[d] = ls1id[i];
-    *src_i = i;
-    *cell_of = c;
-    return d;
 }
 
 __global__ __launch_bounds__(256) void k_bin_gather(int n, MolArrays src, MolArrays dst,
@@ -439,10 +456,12 @@ __global__ __launch_bounds__(256) void k_bin_gather(int n, MolArrays src, MolArr
                                                     const Component *__restrict__ comps,
                                                     int *nsite0, int *nsite1, int *nsite2,
                                                     int *nsite3) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    __shared__ int win[kGatherWin];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
     int i, c;
-    const int d = gather_common(p, src, dst, cell, perm, start, newcell, ls1id, newls1id, &i, &c);
+    const int d = stable_dest(p, n, cell, perm, start, win, &i, &c);
+    if (d < 0) return;
+    gather_common(i, d, c, src, dst, newcell, ls1id, newls1id);
     dst.qw[d] = src.qw[i]; dst.qx[d] = src.qx[i]; dst.qy[d] = src.qy[i]; dst.qz[d] = src.qz[i];
     dst.Lx[d] = src.Lx[i]; dst.Ly[d] = src.Ly[i]; dst.Lz[d] = src.Lz[i];
     const Component &cp = comps[src.cid[i]];
@@ -464,10 +483,12 @@ __global__ __launch_bounds__(256) void k_bin_gather_point(
     const int *__restrict__ perm, const int *__restrict__ start, int *newcell,
     const int *__restrict__ ls1id, int *newls1id, const Component *__restrict__ comps, Reps R,
     float *fx, float *fy, float *fz, int *ljid) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    __shared__ int win[kGatherWin];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
     int i, c;
-    const int d = gather_common(p, src, dst, cell, perm, start, newcell, ls1id, newls1id, &i, &c);
+    const int d = stable_dest(p, n, cell, perm, start, win, &i, &c);
+    if (d < 0) return;
+    gather_common(i, d, c, src, dst, newcell, ls1id, newls1id);
     const int rep = cell_replica(R, c);
     const Geom &g = R.geom[rep];
     const int local = c - R.cell_off[rep];
