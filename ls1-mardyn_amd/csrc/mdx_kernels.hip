@@ -502,13 +502,16 @@ __global__ __launch_bounds__(256) void k_bin_gather_point(
 // q <- N^k(q), N = Quaternion::normalize (q scaled by 1/sqrt(|q|^2)), for
 // molecules whose q is touched by nothing else (one-centre systems, where ls1
 // normalises every molecule at each of the k container updates). N reaches a
-// fixed point or a two-cycle after a few applications; the rest of the k are
-// skipped, or reduced to their parity for a two-cycle.
+// fixed point or a two-cycle after a few applications (|q|^2 is then 1 to
+// within an ulp or two); the rest of the k are skipped, or reduced to their
+// parity for a two-cycle. The loop is capped at kMaxNormalize applications,
+// far beyond where the sequence settles.
+constexpr long long kMaxNormalize = 64;
 __global__ __launch_bounds__(256) void k_normalize_q(int n, MolArrays s, long long k) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]}, qp[4] = {0, 0, 0, 0};
-    for (long long t = 0; t < k; ++t) {
+    for (long long t = 0; t < k && t < kMaxNormalize; ++t) {
         const double qs = 1. / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
         double qn[4];
         for (int c = 0; c < 4; ++c) qn[c] = q[c] * qs;
