@@ -508,6 +508,7 @@ __global__ __launch_bounds__(256) void k_bin_gather_point(
 // far beyond where the sequence settles.
 constexpr long long kMaxNormalize = 64;
 __global__ __launch_bounds__(256) void k_normalize_q(int n, MolArrays s, long long k) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]}, qp[4] = {0, 0, 0, 0};

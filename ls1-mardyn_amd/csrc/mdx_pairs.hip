@@ -378,12 +378,13 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                 r2v[u] = dx * dx + dy * dy + dz * dz;
             }
             if (COUNT && valid) ntest += kUnroll;
+            // every candidate is written at the end of the list and kept only
+            // on a hit (no branch; the drain test above keeps cnt + kUnroll
+            // within the list)
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
-                if (test(r2v[u])) {
-                    T.lst[cnt][lane] = (unsigned short)sv[u];
-                    ++cnt;
-                }
+                T.lst[cnt][lane] = (unsigned short)sv[u];
+                cnt += test(r2v[u]) ? 1 : 0;
             }
         }
         for (; m0 < per; ++m0) {  // the last rows, bounds-tested
@@ -395,10 +396,8 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             const R r2 = dx * dx + dy * dy + dz * dz;
             const bool in = valid && c < nh;
             if (COUNT && in) ++ntest;
-            if (in && test(r2)) {
-                T.lst[cnt][lane] = (unsigned short)sidx;
-                ++cnt;
-            }
+            T.lst[cnt][lane] = (unsigned short)sidx;
+            cnt += (in && test(r2)) ? 1 : 0;
         }
         drain();
         __syncthreads();
@@ -525,7 +524,8 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             for (int k = 0; k < 4; ++k) a.macro_part[4 * (size_t)cell + k] = 0.0;
         return;
     }
-    for (int k = 0; k < 3; ++k)
+    // the reaction-field sum (wmac[2]) is non-zero only for dipole i sites
+    for (int k = 0; k < (TI == T_D ? 3 : 2); ++k)
         for (int o = 32; o > 0; o >>= 1) wmac[k] += shfl_xor_d(wmac[k], o);
     if (lane == 0) {
         double *o = a.macro_part + 4 * (size_t)cell;
