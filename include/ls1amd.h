@@ -185,7 +185,9 @@ int mdx_upload(mdx_context *ctx, int64_t n, const double *r, const double *v, co
 
 /* Copies the state back in upload order. Any pointer may be NULL. F, M and Vi
  * are the molecule force, torque and virial of the last traversal
- * (FullMolecule::calcFM, FullMolecule.cpp:548-631). */
+ * (FullMolecule::calcFM, FullMolecule.cpp:548-631). q is normalised once per
+ * container update since the upload, as setupSoACache does
+ * (FullMolecule.cpp:719). */
 int mdx_download(mdx_context *ctx, int64_t n, double *r, double *v, double *q, double *L,
                  double *F, double *M, double *Vi, uint64_t *ids);
 
@@ -218,7 +220,9 @@ int mdx_leapfrog_post(mdx_context *ctx, double dt, double *sums);
  * the forces of the current state (call mdx_update_container + mdx_traverse
  * once first, as Simulation::prepare_start does). per_step (may be NULL) gets
  * 6 doubles per step: Upot, virial, sum m v^2, sum I w^2, Upot_LJ, Upot_rest.
- * use_graph != 0 replays one captured hipGraph per step. */
+ * use_graph != 0 replays one captured hipGraph per step. Without a thermostat
+ * (mdx_set_thermostat) step n's upd_postF and step n+1's upd_preF run as one
+ * pass over the molecules, with the same results as one-step calls. */
 int mdx_step(mdx_context *ctx, int nsteps, double dt, double *per_step, int use_graph);
 
 /* Blocks until the context's stream is idle. */
