@@ -8,7 +8,7 @@
 //
 // Pair kernels: one 64-lane wavefront per cell. Lane = (i-site, j-phase):
 // the cell's n sites of the i type occupy n lanes per phase and the
-// floor(64/n) phases stride over the j sites of the 27 neighbour cells
+// min(floor(64/n), 8) phases stride over the j sites of the 27 neighbour cells
 // (periodic images through shift vectors, no halo copies). Stage A tests the
 // molecule-centre cutoff of every candidate and appends hits to a per-lane
 // list in LDS; stage B evaluates the site-pair bodies for the dense list, so

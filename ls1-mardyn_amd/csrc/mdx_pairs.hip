@@ -437,11 +437,13 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             }
         }
         for (int chunk = i_begin; chunk < i_end; chunk += 64) {
-            // lane = ph * nI + il: P = floor(64 / nI) phases of the chunk's
-            // nI sites split the partner candidates; the 64 - P * nI lanes
-            // left over idle (phase 0, never a hit)
+            // lane = ph * nI + il: P = min(floor(64 / nI), 8) phases of the
+            // chunk's nI sites split the partner candidates; the 64 - P * nI
+            // lanes left over idle (phase 0, never a hit). Sparse cells have
+            // few candidates, and more than 8 phases would only lengthen the
+            // phase fold (measured)
             const int nI = min(64, i_end - chunk);
-            const int P = 64 / nI;
+            const int P = min(64 / nI, 8);
             const int ph0 = lane / nI, il = lane - ph0 * nI;
             const bool valid = ph0 < P;
             const int ph = valid ? ph0 : 0;
