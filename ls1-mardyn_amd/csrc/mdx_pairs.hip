@@ -360,7 +360,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         auto test = [&](R r2) { return sizeof(R) == 4 ? r2 <= cut : r2 < cut; };
         int m0 = 0;
         for (; m0 + kUnroll <= full; m0 += kUnroll) {
-            // drain when some lane's list could overflow in this round
+            // drain when some lane's list could overflow in this group
             if (__any(cnt > kList<R> - kUnroll)) drain();
             R r2v[kUnroll];
             int sv[kUnroll];

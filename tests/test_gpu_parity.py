@@ -5,7 +5,7 @@ Bars: FP64 form -- forces, torques and virials per molecule within 1e-12
 relative to the largest reference magnitude of the quantity, Upot and virial
 within 1e-12 relative; cell ids and the traversal's cell-pair set bit-exact.
 FP32-pair / FP64-accumulate form -- relative RMS force error <= 1e-5 against
-the FP64 oracle, pair set identical.
+the FP64 oracle (the specified bound), pair set identical.
 """
 import pathlib
 import sys
