@@ -274,9 +274,6 @@ template <> struct Kind<T_Q, T_Q> { static constexpr int k = 6; };
 // reference's halo copy r + L; FP32: + offset of the neighbour's corner).
 // One 16-byte (FP32) or 32-byte (FP64) record per site and field, so a site
 // is fetched with one or two ds_read_b128.
-// per-lane hit list entries: 24 for FP32, 16 for FP64 (measured best; the
-// shorter FP64 list keeps its larger tile within the LDS budget)
-template <typename R> constexpr int kList = sizeof(R) == 8 ? 16 : 24;
 
 template <typename R> struct alignas(4 * sizeof(R)) Q4 {
     R x, y, z, w;
@@ -286,14 +283,17 @@ template <int TI, typename R, bool MOLSITE, bool MULTI> struct TileCfg {
     static constexpr bool ORI = TI != T_LJ;  // partners may be dipoles / quadrupoles
     static constexpr bool MOL = !MOLSITE;    // molecule positions differ from sites
     static constexpr bool LJID = TI == T_LJ && MULTI;  // several LJ centre types
-    // Sites per LDS chunk. The kernels are bound by VALU issue, so the tile
-    // is sized for occupancy first (160 KB of LDS per CU shared by the
-    // wavefronts of its 4 SIMDs, next to the register limit of each
-    // instantiation) and chunk count second; values measured best on the
-    // 4 M-molecule LJ slab (one-centre) and the 1 M-molecule multi-site
-    // mixture (2CLJQ + water).
+    // Per-lane hit list entries (LIST) and sites per LDS chunk (CAP). The
+    // kernels are bound by VALU issue, so the tile is sized for occupancy
+    // first (160 KB of LDS per CU shared by the wavefronts of its 4 SIMDs,
+    // next to the register limit of each instantiation) and chunk count
+    // second; values measured best on the 4 M-molecule LJ slab (one-centre)
+    // and the 1 M-molecule multi-site mixture (2CLJQ + water). The FP64
+    // tiles that carry molecule positions or orientations keep 16-entry
+    // lists for their larger records.
+    static constexpr int LIST = (sizeof(R) == 8 && !(TI == T_LJ && MOLSITE)) ? 16 : 24;
     static constexpr int CAP =
-        TI == T_LJ ? (MOLSITE ? (sizeof(R) == 4 ? 352 : 224) : (sizeof(R) == 4 ? 192 : 128))
+        TI == T_LJ ? (MOLSITE ? (sizeof(R) == 4 ? 352 : 192) : (sizeof(R) == 4 ? 192 : 128))
         : TI == T_C ? (sizeof(R) == 4 ? 112 : 96)
                     : (sizeof(R) == 4 ? 192 : 160);
 };
@@ -306,7 +306,7 @@ template <int TI, typename R, bool MOLSITE, bool MULTI> struct Tile {
     // first-site (head) indices of a tile of charges (partners tested per molecule)
     unsigned short h[C::MOL && TI != T_LJ ? C::CAP : 1];
     Q4<R> e[C::ORI ? C::CAP : 1];      // orientation (dipole / quadrupole axis)
-    unsigned short lst[kList<R>][64];  // per-lane hit lists (tile indices)
+    unsigned short lst[C::LIST][64];  // per-lane hit lists (tile indices)
     int j0[27], pre[28];
     double shift[27][3];
 };

@@ -361,7 +361,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         int m0 = 0;
         for (; m0 + kUnroll <= full; m0 += kUnroll) {
             // drain when some lane's list could overflow in this group
-            if (__any(cnt > kList<R> - kUnroll)) drain();
+            if (__any(cnt > C::LIST - kUnroll)) drain();
             R r2v[kUnroll];
             int sv[kUnroll];
 #pragma unroll
@@ -388,7 +388,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             }
         }
         for (; m0 < per; ++m0) {  // the last rows, bounds-tested
-            if (__any(cnt > kList<R> - 1)) drain();
+            if (__any(cnt > C::LIST - 1)) drain();
             const int c = ph + m0 * P;
             const int sidx = cand(min(c, nh - 1));
             const Q4<R> q = C::MOL ? T.m[sidx] : T.p[sidx];
