@@ -96,7 +96,7 @@ def run_gpu(s, precision, steps, warmup, kernel_reps, rmm=False):
         ctx.set_counting(False)
         phase("counted traversal")
         if PAIR_SHAPE is None:
-            shape, _ = ctx.autotune(reps=2)
+            shape, _ = ctx.autotune(reps=5)
             phase(f"autotuned launch shape (cells per block, blocks per XCD) = {shape}")
         else:
             shape = PAIR_SHAPE

@@ -118,8 +118,9 @@ int mdx_set_pair_config(mdx_context *ctx, int cells_per_block, int blocks_per_xc
 
 /* Times the pair kernels of the current state over the candidate launch
  * shapes (cells_per_block 1, 2, 4, 8 x blocks_per_xcd 0, 64, 128, 256, 512),
- * reps timed traversals each (HIP events), and keeps the fastest. chosen
- * (may be NULL) = {cells_per_block, blocks_per_xcd}; best_ms = its time. */
+ * reps timed traversals each (HIP events; a shape's time is its fastest
+ * run), and keeps the fastest shape. chosen (may be NULL) =
+ * {cells_per_block, blocks_per_xcd}; best_ms = its time. */
 int mdx_autotune(mdx_context *ctx, int reps, int32_t *chosen, double *best_ms);
 
 /* The NVT tail of Simulation::simulateOneTimestep (Simulation.cpp:1161-1266)

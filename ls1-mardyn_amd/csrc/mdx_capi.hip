@@ -1261,8 +1261,7 @@ int mdx_autotune(mdx_context *c, int reps, int32_t *chosen, double *best_ms) {
     c->counting = false;
     const int cpbs[] = {1, 2, 4, 8};
     const int bpx[] = {0, 64, 128, 256, 512};
-    double best = 1e300;
-    int bc = c->cpb, bb = c->blocks_per_xcd;
+    std::vector<std::pair<int, int>> cand;
     for (int cp : cpbs)
         for (int b : bpx) {
             c->cpb = cp;
@@ -1270,21 +1269,30 @@ int mdx_autotune(mdx_context *c, int reps, int32_t *chosen, double *best_ms) {
             int ipx = 0;
             (void)pair_grid(c, &ipx);
             if (b > 0 && b >= ipx) continue;  // same as one block per item
-            double t = 0;
-            for (int k = 0; k <= reps; ++k) {
-                HIPCHK(c, hipMemsetAsync(c->step_idx, 0, sizeof(int), c->stream));
-                TRY(launch_traverse(c));
-                HIPCHK(c, hipStreamSynchronize(c->stream));
-                float ms = 0;
-                HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-                if (k > 0) t += ms;  // first run warms up
-            }
-            t /= reps;
-            if (t < best) {
-                best = t;
-                bc = cp;
-                bb = b;
-            }
+            cand.push_back({cp, b});
+        }
+    // rounds over all candidates (the first warms up); a shape's time is its
+    // fastest run, the least noisy estimate, and interleaving the candidates
+    // keeps slow drifts of the clock from favouring the ones timed first
+    std::vector<double> t(cand.size(), 1e300);
+    for (int k = 0; k <= reps; ++k)
+        for (size_t i = 0; i < cand.size(); ++i) {
+            c->cpb = cand[i].first;
+            c->blocks_per_xcd = cand[i].second;
+            HIPCHK(c, hipMemsetAsync(c->step_idx, 0, sizeof(int), c->stream));
+            TRY(launch_traverse(c));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            float ms = 0;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            if (k > 0) t[i] = std::min(t[i], (double)ms);
+        }
+    double best = 1e300;
+    int bc = c->cpb, bb = c->blocks_per_xcd;
+    for (size_t i = 0; i < cand.size(); ++i)
+        if (t[i] < best) {
+            best = t[i];
+            bc = cand[i].first;
+            bb = cand[i].second;
         }
     c->cpb = bc;
     c->blocks_per_xcd = bb;
