@@ -60,7 +60,7 @@ def test_batch_equals_single_systems(prec):
                                    atol=1e-11 * np.abs(p1[:, :4]).max())
 
 
-@pytest.mark.parametrize("shape", [(2, 0), (4, 16), (1, 8), (8, 64)])
+@pytest.mark.parametrize("shape", [(2, 0), (4, 16), (1, 8), (8, 64), (16, 0), (16, 32)])
 def test_launch_shapes_bit_identical(shape):
     s = scenarios.small_mixture(n=400)
     print("case shape", shape, flush=True)
