@@ -403,8 +403,8 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "copies": 3}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path; run with "
+              "--gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "adjoint":
