@@ -200,7 +200,8 @@ int mdx_download(mdx_context *ctx, int64_t n, double *r, double *v, double *q, d
 int mdx_update_container(mdx_context *ctx);
 
 /* Force traversal: every site pair of every molecule pair closer than the
- * cutoff, then FullMolecule::calcFM. Replaces LinkedCells::traverseCells with
+ * cutoff, then FullMolecule::calcFM. A state not binned since its upload is
+ * binned first (as are mdx_autotune's, mdx_adjoint's and mdx_cell_ids'). Replaces LinkedCells::traverseCells with
  * VectorizedCellProcessor (LinkedCells.cpp:565-575: initTraversal,
  * processCell/processCellPair over the C08 enumeration, endTraversal) and
  * Simulation::updateForces (Simulation.cpp:835-845). macro may be NULL. */

@@ -80,6 +80,9 @@ struct mdx_context {
     // normalisation of q (setupSoACache) is applied when q is read
     // (q_renorm = container updates since the last application)
     int64_t q_renorm = 0;
+    // the cell lists (site starts, site cache) describe the current state:
+    // false after an upload until the first container update
+    bool binned = false;
     // NVT tail of the step (Domain::calculateGlobalValues, Homogeneous LRC,
     // velocity-scaling thermostat), off by default (the NVE path)
     bool lrc = false, thermostat = false;
@@ -723,6 +726,7 @@ int launch_update(mdx_context *c) {
                            dst, c->cell, c->perm, c->start, c->mcell, c->ls1id, c->mls1id,
                            c->d_comps, reps(c), s.fx, s.fy, s.fz, s.ljid);
         c->cur = 1 - c->cur;
+        c->binned = true;
         HIPCHK(c, hipGetLastError());
         return MDX_OK;
     }
@@ -740,7 +744,15 @@ int launch_update(mdx_context *c) {
     hipLaunchKernelGGL(k_build_sites, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, c->mol[c->cur],
                        reps(c), c->mcell, c->d_comps, c->sites[0], c->sites[1], c->sites[2],
                        c->sites[3]);
+    c->binned = true;
     HIPCHK(c, hipGetLastError());
+    return MDX_OK;
+}
+
+// Traversals read the cell lists: a state not binned since its upload is
+// binned first (LinkedCells' update before the first traversal).
+int ensure_binned(mdx_context *c) {
+    if (!c->binned) TRY(launch_update(c));
     return MDX_OK;
 }
 
@@ -919,6 +931,7 @@ int state_setup(mdx_context *c, int64_t n64, const int32_t *cid_host) {
     }
     for (int t = 0; t < kTypes; ++t)
         if (c->nsites[t] >= (1 << 27)) return fail(c, MDX_ERR_CAPACITY, "too many sites");
+    c->binned = false;
     TRY(alloc_mol(c, c->mol[0], n));
     TRY(alloc_mol(c, c->mol[1], n, c->point_lj ? &c->mol[0] : nullptr));
     c->q_renorm = 0;
@@ -1255,6 +1268,7 @@ int mdx_autotune(mdx_context *c, int reps, int32_t *chosen, double *best_ms) {
     if (!c) return MDX_ERR_ARG;
     TRY(need(c, true, true, true));
     HIPCHK(c, hipSetDevice(c->device));
+    TRY(ensure_binned(c));
     if (reps < 1) reps = 3;
     const bool was_timing = c->timing, was_counting = c->counting;
     c->timing = true;
@@ -1472,6 +1486,7 @@ int mdx_traverse(mdx_context *c, double *macro) {
     if (!c) return MDX_ERR_ARG;
     TRY(need(c, true, true, true));
     HIPCHK(c, hipSetDevice(c->device));
+    TRY(ensure_binned(c));
     if (c->counting)
         HIPCHK(c, hipMemsetAsync(c->d_counters, 0, 8 * sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipMemsetAsync(c->step_idx, 0, sizeof(int), c->stream));
@@ -1586,6 +1601,8 @@ int mdx_cell_ids(mdx_context *c, int64_t n, int64_t *out) {
     TRY(need(c, true, true, true));
     if (c->nrep > 1) return fail(c, MDX_ERR_ARG, "not available for batched replicas");
     if (n != c->n) return fail(c, MDX_ERR_ARG, "n differs from the uploaded count");
+    HIPCHK(c, hipSetDevice(c->device));
+    TRY(ensure_binned(c));
     std::vector<int> tmp((size_t)n), orig((size_t)n);
     HIPCHK(c, hipMemcpyAsync(tmp.data(), c->mls1id, (size_t)n * 4, hipMemcpyDeviceToHost,
                              c->stream));
@@ -1692,6 +1709,8 @@ int mdx_adjoint(mdx_context *c, int64_t n, const double *cot, double *dU_deps, d
     if (c->nrep > 1) return fail(c, MDX_ERR_ARG, "not available for batched replicas");
     if (c->nsites[T_D] > 0 || c->nsites[T_Q] > 0)
         return fail(c, MDX_ERR_ARG, "the adjoint covers LJ and charge sites only");
+    HIPCHK(c, hipSetDevice(c->device));
+    TRY(ensure_binned(c));
     const int nl = c->model.nlj;
     if (nl > kAdjMaxLJ) return fail(c, MDX_ERR_CAPACITY, "the adjoint supports <= 16 LJ centres");
     HIPCHK(c, hipSetDevice(c->device));

@@ -1,0 +1,56 @@
+"""Calls that read the cell lists right after an upload (no
+mdx_update_container first) bin the state themselves: traversal, autotuner
+and cell ids give what the documented order (upload, update, traverse)
+gives."""
+import sys
+
+import numpy as np
+import pytest
+
+import scenarios
+from ls1amd import PREC_FP32_FP64ACC, PREC_FP64, Context
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("sysname", ["small_lj", "small_mixture"])
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_FP32_FP64ACC])
+def test_traverse_right_after_upload(sysname, precision):
+    say("api order", sysname, precision)
+    s = getattr(scenarios, sysname)(seed=91)
+    with Context(0) as c:
+        c.setup_system(s, periodic=True, precision=precision)
+        c.update_container()
+        m_ref = c.traverse()
+        ref = c.download()
+    with Context(0) as c:
+        c.setup_system(s, periodic=True, precision=precision)
+        m = c.traverse()
+        out = c.download()
+    np.testing.assert_array_equal(m, m_ref)
+    for key in ("F", "M", "Vi"):
+        assert np.array_equal(out[key], ref[key]), key
+
+
+def test_autotune_and_cell_ids_right_after_upload():
+    say("api order autotune")
+    s = scenarios.small_lj(seed=92)
+    o = Oracle(s.ncomp, s.comp_desc, s.mix, s.eps_rf, s.rc, s.rc_lj, s.box, periodic=True)
+    with Context(0) as c:
+        c.setup_system(s, periodic=True, precision=PREC_FP64)
+        ids = c.cell_ids()
+    np.testing.assert_array_equal(ids, o.cell_index(s.r))
+    with Context(0) as c:
+        c.setup_system(s, periodic=True, precision=PREC_FP64)
+        (cpb, bpx), ms = c.autotune(reps=1)
+        assert 1 <= cpb <= 8 and ms > 0
+        c.traverse()
+        out = c.download()
+    ref = o.forces(s.r, s.q, s.cid)
+    assert np.abs(out["F"] - ref["F"]).max() < 1e-12 * np.abs(ref["F"]).max()
