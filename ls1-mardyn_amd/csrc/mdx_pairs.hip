@@ -410,7 +410,6 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
 }
 
 __device__ __forceinline__ double shfl_xor_d(double v, int o) { return __shfl_xor(v, o, 64); }
-__device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
 
 // All i sites of one cell (global index) with their partners; writes the
 // cell's macroscopic sums to macro_part[4 * cell].
@@ -478,23 +477,31 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
                                                                     lane, T, acc, eps24, sig2,
                                                                     shift6, ntest);
             }
-            // fold the phases of each site into its phase-0 lane (tree over
-            // the phases: after the step of width w, lane ph holds phases
-            // ph .. ph + 2w - 1; the virial sums below take the per-lane
-            // parts before the fold)
+            // fold the phases of each site into its phase-0 lane, in phase
+            // order (the virial sums below take the per-lane parts before
+            // the fold)
             wmac[1] += acc.v[0] + acc.v[1] + acc.v[2];  // virial = trace of the site tensors
-            for (int w = 1; w < P; w <<= 1) {
-                const bool take = valid && ph + w < P;
-                const int src = take ? lane + w * nI : lane;
+            if (P > 1) {
+                // through LDS: the tile is free after the partner passes
+                constexpr int NA = TI >= T_D ? 9 : 6;
+                static_assert(sizeof(T) >= NA * 64 * sizeof(double), "fold scratch");
+                double *fb = reinterpret_cast<double *>(&T);
                 for (int k = 0; k < 3; ++k) {
-                    const double f = shfl_d(acc.f[k], src), v = shfl_d(acc.v[k], src);
-                    acc.f[k] += take ? f : 0.0;
-                    acc.v[k] += take ? v : 0.0;
-                    if constexpr (TI >= T_D) {
-                        const double m = shfl_d(acc.m[k], src);
-                        acc.m[k] += take ? m : 0.0;
-                    }
+                    fb[k * 64 + lane] = acc.f[k];
+                    fb[(3 + k) * 64 + lane] = acc.v[k];
+                    if constexpr (TI >= T_D) fb[(6 + k) * 64 + lane] = acc.m[k];
                 }
+                __syncthreads();
+                if (valid && ph == 0)
+                    for (int q = 1; q < P; ++q) {
+                        const int src = lane + q * nI;
+                        for (int k = 0; k < 3; ++k) {
+                            acc.f[k] += fb[k * 64 + src];
+                            acc.v[k] += fb[(3 + k) * 64 + src];
+                            if constexpr (TI >= T_D) acc.m[k] += fb[(6 + k) * 64 + src];
+                        }
+                    }
+                __syncthreads();
             }
             if (valid && ph == 0 && a.kvx) {
                 // VCP1CLJRMM (VCP1CLJRMM.cpp:300-362): the site's force, times
