@@ -16,7 +16,7 @@
 // accumulates only its own force, virial and torque (owner computes), so no
 // atomics and no colouring are needed and results are deterministic; each
 // unordered pair is evaluated from both sides and the macroscopic sums are
-// halved. Phases are folded with cross-lane shuffles at the end of the cell.
+// halved. Phases are folded through LDS at the end of the cell.
 #include "mdx_internal.h"
 
 namespace mdx {
