@@ -122,7 +122,9 @@ struct PairArgs {
     int cpb;            // cells per work item (consecutive cells, one block)
     int items_per_xcd;  // work items of each XCD's contiguous range
     int present[kTypes];
-    double *macro_part; // [ncell][4] per-cell sums {u6 (LJ), uXpoles, virial, rf}
+    double *macro_part; // [ncell][4] sums {u6 (LJ), uXpoles, virial, rf}: each work item
+                        // stores its cells' sums in the row of its last non-empty cell
+                        // (per replica) and zeros in the others
     int accumulate;     // add to macro_part rows (a later i-site type) instead of writing
     double vscale;      // factor on the written site virial: 1, or 0.5 where the site
                         // arrays are the molecule arrays (calcFM's Vi = 0.5 sum V_site)

@@ -411,12 +411,14 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
 
 __device__ __forceinline__ double shfl_xor_d(double v, int o) { return __shfl_xor(v, o, 64); }
 
-// All i sites of one cell (global index) with their partners; writes the
-// cell's macroscopic sums to macro_part[4 * cell].
+// All i sites of one cell (global index) with their partners; adds the
+// cell's per-lane macroscopic sums to wmac and writes a zero row for the cell
+// (flush_macro later stores the sums of a group of cells in one row).
+// Returns whether the cell has i sites.
 template <int TI, typename R, bool COUNT, bool MOLSITE, bool MULTI>
-__device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE, MULTI> &T,
-                                          int cell, int lane, unsigned long long &ntest) {
-    double wmac[3] = {0, 0, 0};
+__device__ __forceinline__ bool pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE, MULTI> &T,
+                                          int cell, int lane, unsigned long long &ntest,
+                                          double (&wmac)[3]) {
     bool empty;
     {
         const int rep = cell_replica(a.R, cell);
@@ -528,11 +530,17 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             wmac[2] += acc.rf;
         }
     }
-    if (empty) {  // zero row; nothing to add to an existing one
-        if (lane == 0 && !a.accumulate)
-            for (int k = 0; k < 4; ++k) a.macro_part[4 * (size_t)cell + k] = 0.0;
-        return;
-    }
+    // a zero row (the sums of the work item's cells go to one row at the end)
+    if (lane == 0 && !a.accumulate)
+        for (int k = 0; k < 4; ++k) a.macro_part[4 * (size_t)cell + k] = 0.0;
+    return !empty;
+}
+
+// The per-lane sums of a group of cells of one replica into the row of the
+// group's last non-empty cell.
+template <int TI>
+__device__ __forceinline__ void flush_macro(const PairArgs &a, double (&wmac)[3], int cell,
+                                            int lane) {
     // the reaction-field sum (wmac[2]) is non-zero only for dipole i sites
     for (int k = 0; k < (TI == T_D ? 3 : 2); ++k)
         for (int o = 32; o > 0; o >>= 1) wmac[k] += shfl_xor_d(wmac[k], o);
@@ -542,6 +550,7 @@ __device__ __forceinline__ void pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
                              wmac[2]};
         for (int k = 0; k < 4; ++k) o[k] = a.accumulate ? o[k] + v[k] : v[k];
     }
+    wmac[0] = wmac[1] = wmac[2] = 0.0;
 }
 
 // One 64-lane block (one wavefront) per work item of cpb consecutive cells.
@@ -559,8 +568,18 @@ __global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
     for (int item = blockIdx.x >> 3; item < a.items_per_xcd; item += stride) {
         const int c0 = (xcd * a.items_per_xcd + item) * a.cpb;
         const int c1 = min(c0 + a.cpb, a.ncell);
-        for (int cell = c0; cell < c1; ++cell)
-            pair_cell<TI, R, COUNT, MOLSITE, MULTI>(a, T, cell, lane, ntest);
+        double wmac[3] = {0.0, 0.0, 0.0};
+        int last = -1, rep_prev = -1;
+        for (int cell = c0; cell < c1; ++cell) {
+            const int rep = cell_replica(a.R, cell);
+            if (rep != rep_prev && last >= 0) {  // rows never mix replicas
+                flush_macro<TI>(a, wmac, last, lane);
+                last = -1;
+            }
+            rep_prev = rep;
+            if (pair_cell<TI, R, COUNT, MOLSITE, MULTI>(a, T, cell, lane, ntest, wmac)) last = cell;
+        }
+        if (last >= 0) flush_macro<TI>(a, wmac, last, lane);
     }
     if (COUNT) {
         for (int o = 32; o > 0; o >>= 1) ntest += __shfl_xor(ntest, o, 64);
