@@ -445,7 +445,10 @@ __device__ __forceinline__ bool pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             // phase fold (measured)
             const int nI = min(64, i_end - chunk);
             const int P = min(64 / nI, 8);
-            const int ph0 = lane / nI, il = lane - ph0 * nI;
+            // lane / nI through a float reciprocal (exact: lane < 64, nI <= 64,
+            // and (lane + 0.5) / nI stays >= 1/128 away from an integer)
+            const int ph0 = (int)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)nI));
+            const int il = lane - ph0 * nI;
             const bool valid = ph0 < P;
             const int ph = valid ? ph0 : 0;
             const int i = chunk + (valid ? il : 0);
