@@ -380,11 +380,19 @@ int make_geom(mdx_context *c, const double *box, double rc, int periodic, int ce
         g.L[d] = box[d];
         double rcc = cell_mode ? (double)(float)rc : rc;
         int nb = (int)floor(box[d] / rcc);
-        if (nb == 0) nb = 1;
-        if (nb < 2)
-            return fail(c, MDX_ERR_ARG,
-                        "bounding box too small for the cell length (LinkedCells needs at "
-                        "least 2 cells per dimension, LinkedCells.cpp:79-96)");
+        if (cell_mode) {
+            // rebuild: one cell per dimension is allowed (the halo images are
+            // then copies of that cell), none is not (LinkedCells.cpp:153-164)
+            if (nb == 0)
+                return fail(c, MDX_ERR_ARG,
+                            "region too small: no cell of edge rc fits (LinkedCells.cpp:160-164)");
+        } else {
+            if (nb == 0) nb = 1;
+            if (nb < 2)
+                return fail(c, MDX_ERR_ARG,
+                            "bounding box too small for the cell length (LinkedCells needs at "
+                            "least 2 cells per dimension, LinkedCells.cpp:79-96)");
+        }
         g.n[d] = nb;
         g.cpd[d] = nb + 2;
         g.cl[d] = box[d] / nb;

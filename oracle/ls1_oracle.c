@@ -218,12 +218,14 @@ static void build_lj_tables(ox_ctx *x, const double *mix) {
 }
 
 /* LinkedCells::LinkedCells (cell_mode 0, double rc; LinkedCells.cpp:38-106)
- * and LinkedCells::rebuild (cell_mode 1, float rc; LinkedCells.cpp:132-201). */
-static void build_geometry(ox_ctx *x, int cell_mode) {
+ * and LinkedCells::rebuild (cell_mode 1, float rc; LinkedCells.cpp:132-201).
+ * Returns -1 where rebuild exits (no cell fits, LinkedCells.cpp:160-164). */
+static int build_geometry(ox_ctx *x, int cell_mode) {
     for (int d = 0; d < 3; ++d) {
         double diff = x->bbmax[d] - x->bbmin[d];
         double rcc = cell_mode ? (double)(float)x->rc : x->rc;
         int nb = (int)floor(diff / rcc);
+        if (nb == 0 && cell_mode) return -1;
         if (nb == 0) nb = 1;
         x->boxcells[d] = nb;
         x->cell_len[d] = diff / nb;
@@ -232,6 +234,7 @@ static void build_geometry(ox_ctx *x, int cell_mode) {
         x->halo_max[d] = x->bbmax[d] + x->cell_len[d];
         x->cpd[d] = nb + 2;
     }
+    return 0;
 }
 
 void *ox_create(int ncomp, const double *desc, int desc_len, const double *mix,
@@ -258,7 +261,10 @@ void *ox_create(int ncomp, const double *desc, int desc_len, const double *mix,
         x->wall[d] = 0;
     }
     build_lj_tables(x, mix);
-    build_geometry(x, cell_mode);
+    if (build_geometry(x, cell_mode) < 0) {
+        ox_destroy(x);
+        return NULL;
+    }
     return x;
 }
 
