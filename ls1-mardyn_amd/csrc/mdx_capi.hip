@@ -56,7 +56,9 @@ namespace {
 thread_local std::string g_create_error;
 constexpr int kBlock = 256;
 inline int nblk(long n, int b = kBlock) { return (int)std::max<long>(1, (n + b - 1) / b); }
+
 }  // namespace
+
 
 struct mdx_context {
     int device = 0;
@@ -1708,8 +1710,21 @@ int mdx_set_charges(mdx_context *c, int ncomp, const double *q) {
     return MDX_OK;
 }
 
-int mdx_adjoint(mdx_context *c, int64_t n, const double *cot, double *dU_deps, double *dU_dsig,
-                double *dW_deps, double *dW_dsig, double *dU_dq, double *dW_dq, double *vjp) {
+}  // extern "C"
+
+namespace {
+
+// Device buffers of one adjoint evaluation.
+struct AdjBufs {
+    double *site;  // 3 (LJ sites + charge sites): per-site VJP terms
+    double *cot;   // 3 n: the cotangent in the current molecule order
+    double *part;  // grid x (largest K * W): per-block parameter tables
+    double *out;   // kAdjMaxLJ^2 * 4 + kMaxComp * kMaxSite * 2: reduced tables
+};
+
+constexpr size_t kAdjOutDoubles = (size_t)kAdjMaxLJ * kAdjMaxLJ * 4 + kMaxComp * kMaxSite * 2;
+
+int adjoint_check(mdx_context *c, int64_t n, const double *cot, const double *vjp) {
     if (!c) return MDX_ERR_ARG;
     TRY(need(c, true, true, true));
     if (n != c->n) return fail(c, MDX_ERR_ARG, "n differs from the uploaded count");
@@ -1717,30 +1732,39 @@ int mdx_adjoint(mdx_context *c, int64_t n, const double *cot, double *dU_deps, d
     if (c->nrep > 1) return fail(c, MDX_ERR_ARG, "not available for batched replicas");
     if (c->nsites[T_D] > 0 || c->nsites[T_Q] > 0)
         return fail(c, MDX_ERR_ARG, "the adjoint covers LJ and charge sites only");
+    if (c->model.nlj > kAdjMaxLJ)
+        return fail(c, MDX_ERR_CAPACITY, "the adjoint supports <= 16 LJ centres");
+    return MDX_OK;
+}
+
+int adjoint_grid(const mdx_context *c) { return std::max(1, std::min(c->ncell, kAdjBlocks)); }
+
+// Largest per-block table (keys x values) of the LJ and charge passes.
+size_t adjoint_table_width(const mdx_context *c) {
+    const size_t nl = c->model.nlj;
+    return std::max(nl * 4 * nl, (size_t)c->model.ncomp * kMaxSite * 2);
+}
+
+// The adjoint on the given buffers (no allocation): the LJ and charge
+// passes, the per-molecule VJP, the copies back and the symmetric sums.
+int adjoint_run(mdx_context *c, const double *cot, double *dU_deps, double *dU_dsig,
+                double *dW_deps, double *dW_dsig, double *dU_dq, double *dW_dq, double *vjp,
+                const AdjBufs &b) {
     HIPCHK(c, hipSetDevice(c->device));
     TRY(ensure_binned(c));
     const int nl = c->model.nlj;
-    if (nl > kAdjMaxLJ) return fail(c, MDX_ERR_CAPACITY, "the adjoint supports <= 16 LJ centres");
-    HIPCHK(c, hipSetDevice(c->device));
     const int nn = c->n, ncomp = c->model.ncomp;
     const int nsl = c->nsites[T_LJ], nsc = c->nsites[T_C];
-    if (!c->adj_site) {
-        TRY(dalloc(c, &c->adj_site, 3 * ((size_t)nsl + nsc), true));
-        TRY(dalloc(c, &c->adj_cot, 3 * (size_t)nn, true));
-        TRY(dalloc(c, &c->adj_part, (size_t)kAdjBlocks * kAdjMaxLJ * kAdjMaxLJ * 4, true));
-        TRY(dalloc(c, &c->adj_out, (size_t)kAdjMaxLJ * kAdjMaxLJ * 4 + kMaxComp * kMaxSite * 2,
-                   true));
-    }
     MolArrays &m = c->mol[c->cur];
     if (cot) {
         HIPCHK(c, hipMemcpyAsync(c->dl_tmp, cot, (size_t)nn * 3 * 8, hipMemcpyHostToDevice,
                                  c->stream));
         hipLaunchKernelGGL(gather_orig3_kernel(), dim3(nblk(nn)), dim3(kBlock), 0, c->stream, nn,
-                           m.orig, c->dl_tmp, c->adj_cot);
+                           m.orig, c->dl_tmp, b.cot);
     }
-    const int grid = std::max(1, std::min(c->ncell, kAdjBlocks));
-    double *out_lj = c->adj_out, *out_q = c->adj_out + (size_t)kAdjMaxLJ * kAdjMaxLJ * 4;
-    double *g_site[2] = {c->adj_site, c->adj_site + 3 * (size_t)nsl};
+    const int grid = adjoint_grid(c);
+    double *out_lj = b.out, *out_q = b.out + (size_t)kAdjMaxLJ * kAdjMaxLJ * 4;
+    double *g_site[2] = {b.site, b.site + 3 * (size_t)nsl};
     const int ts[2] = {T_LJ, T_C};
     for (int u = 0; u < 2; ++u) {
         const int t = ts[u];
@@ -1756,15 +1780,15 @@ int mdx_adjoint(mdx_context *c, int64_t n, const double *cot, double *dU_deps, d
         a.shift_rc = c->d_shift_rc;
         a.mcid = m.cid;
         if (cot) {
-            a.cx = c->adj_cot;
-            a.cy = c->adj_cot + nn;
-            a.cz = c->adj_cot + 2 * (size_t)nn;
+            a.cx = b.cot;
+            a.cy = b.cot + nn;
+            a.cz = b.cot + 2 * (size_t)nn;
             const size_t ns = c->nsites[t];
             a.gx = g_site[u];
             a.gy = g_site[u] + ns;
             a.gz = g_site[u] + 2 * ns;
         }
-        a.part = c->adj_part;
+        a.part = b.part;
         a.nlj = nl;
         a.ncell = c->ncell;
         a.K = t == T_LJ ? nl : ncomp * kMaxSite;
@@ -1772,13 +1796,17 @@ int mdx_adjoint(mdx_context *c, int64_t n, const double *cot, double *dU_deps, d
         const size_t lds = ((size_t)a.W * 64 + (size_t)a.K * a.W) * 8 + 64 * 4;
         hipLaunchKernelGGL(adjoint_kernel(t), dim3(grid), dim3(64), lds, c->stream, a);
         const int width = a.K * a.W;
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->stream, c->adj_part, grid,
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->stream, b.part, grid,
                            width, grid, t == T_LJ ? out_lj : out_q);
     }
     HIPCHK(c, hipGetLastError());
-    std::vector<double> hl((size_t)nl * nl * 4, 0.0), hq((size_t)ncomp * kMaxSite * 2, 0.0);
-    if (nsl) HIPCHK(c, hipMemcpyAsync(hl.data(), out_lj, hl.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    if (nsc) HIPCHK(c, hipMemcpyAsync(hq.data(), out_q, hq.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    // host tables on the stack (no allocation): at most kAdjMaxLJ LJ ids
+    double hl[kAdjMaxLJ * kAdjMaxLJ * 4], hq[kMaxComp * kMaxSite * 2];
+    const size_t nhl = (size_t)nl * nl * 4, nhq = (size_t)ncomp * kMaxSite * 2;
+    std::fill(hl, hl + nhl, 0.0);
+    std::fill(hq, hq + nhq, 0.0);
+    if (nsl) HIPCHK(c, hipMemcpyAsync(hl, out_lj, nhl * 8, hipMemcpyDeviceToHost, c->stream));
+    if (nsc) HIPCHK(c, hipMemcpyAsync(hq, out_q, nhq * 8, hipMemcpyDeviceToHost, c->stream));
     if (vjp) {
         const size_t s0 = nsl, s1 = nsc;
         hipLaunchKernelGGL(adjoint_mol_kernel(), dim3(nblk(nn)), dim3(kBlock), 0, c->stream, nn,
@@ -1787,8 +1815,8 @@ int mdx_adjoint(mdx_context *c, int64_t n, const double *cot, double *dU_deps, d
                            nsc ? g_site[1] : nullptr, g_site[1] + s1, g_site[1] + 2 * s1,
                            c->dl_tmp);
         hipLaunchKernelGGL(k_scatter_orig, dim3(nblk(nn)), dim3(kBlock), 0, c->stream, nn, m.orig,
-                           c->dl_tmp, c->adj_cot, nn, 3);
-        HIPCHK(c, hipMemcpyAsync(vjp, c->adj_cot, (size_t)nn * 3 * 8, hipMemcpyDeviceToHost,
+                           c->dl_tmp, b.cot, nn, 3);
+        HIPCHK(c, hipMemcpyAsync(vjp, b.cot, (size_t)nn * 3 * 8, hipMemcpyDeviceToHost,
                                  c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1809,6 +1837,25 @@ int mdx_adjoint(mdx_context *c, int64_t n, const double *cot, double *dU_deps, d
         if (dW_dq) dW_dq[k] = hq[2 * k + 1];
     }
     return MDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdx_adjoint(mdx_context *c, int64_t n, const double *cot, double *dU_deps, double *dU_dsig,
+                double *dW_deps, double *dW_dsig, double *dU_dq, double *dW_dq, double *vjp) {
+    TRY(adjoint_check(c, n, cot, vjp));
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->adj_site) {
+        const size_t nsl = c->nsites[T_LJ], nsc = c->nsites[T_C];
+        TRY(dalloc(c, &c->adj_site, 3 * (nsl + nsc), true));
+        TRY(dalloc(c, &c->adj_cot, 3 * (size_t)c->n, true));
+        TRY(dalloc(c, &c->adj_part, (size_t)kAdjBlocks * kAdjMaxLJ * kAdjMaxLJ * 4, true));
+        TRY(dalloc(c, &c->adj_out, kAdjOutDoubles, true));
+    }
+    return adjoint_run(c, cot, dU_deps, dU_dsig, dW_deps, dW_dsig, dU_dq, dW_dq, vjp,
+                       AdjBufs{c->adj_site, c->adj_cot, c->adj_part, c->adj_out});
 }
 
 int mdx_set_counting(mdx_context *c, int enable) {

@@ -16,18 +16,31 @@ void HIPLinkedCells::check(int rc, const char *what) {
 HIPLinkedCells::HIPLinkedCells(double bBoxMin[3], double bBoxMax[3], double cutoffRadius,
                                double LJcutoffRadius, const ComponentModel &model, bool periodic,
                                int device, int precision)
-    : _model(model), _cutoff(cutoffRadius), _ljCutoff(LJcutoffRadius), _periodic(periodic) {
+    : _model(model), _cutoff(cutoffRadius), _ljCutoff(LJcutoffRadius), _periodic(periodic),
+      _device(device), _precision(precision) {
     for (int d = 0; d < 3; ++d) {
         _bbMin[d] = bBoxMin[d];
         _bbMax[d] = bBoxMax[d];
     }
-    if (mdx_create(device, &_ctx) != MDX_OK)
+    makeContext(0);
+}
+
+// A context on the container's device with its model, domain, precision and
+// the thermostat / boundary settings made through this class.
+void HIPLinkedCells::makeContext(int cellMode) {
+    if (_ctx) mdx_destroy(_ctx);
+    _ctx = nullptr;
+    if (mdx_create(_device, &_ctx) != MDX_OK)
         die(std::string("mdx_create failed: ") + mdx_last_error(nullptr));
-    check(mdx_set_components(_ctx, model.ncomp, model.desc.data(), (int)model.desc.size(),
-                             model.mix.empty() ? nullptr : model.mix.data(), model.epsilonRF),
+    check(mdx_set_components(_ctx, _model.ncomp, _model.desc.data(), (int)_model.desc.size(),
+                             _model.mix.empty() ? nullptr : _model.mix.data(), _model.epsilonRF),
           "mdx_set_components");
-    setDomain(0);
-    check(mdx_set_precision(_ctx, precision), "mdx_set_precision");
+    setDomain(cellMode);
+    check(mdx_set_precision(_ctx, _precision), "mdx_set_precision");
+    if (_haveBoundaries) check(mdx_set_boundaries(_ctx, _boundaries), "mdx_set_boundaries");
+    if (_haveThermostat)
+        check(mdx_set_thermostat(_ctx, 1, &_targetT, _lrc ? 1 : 0, _heuristics ? 1 : 0),
+              "mdx_set_thermostat");
 }
 
 HIPLinkedCells::~HIPLinkedCells() { mdx_destroy(_ctx); }
@@ -41,33 +54,49 @@ void HIPLinkedCells::setDomain(int cellMode) {
           "mdx_set_domain");
 }
 
-// LinkedCells::rebuild (LinkedCells.cpp:132-201) uses a float cell edge.
+// LinkedCells::rebuild (LinkedCells.cpp:132-201) uses a float cell edge. The
+// molecules are kept; the new context keeps the device, precision and the
+// settings made through this class (settings made directly on context() are
+// not carried over).
 bool HIPLinkedCells::rebuild(double bBoxMin[3], double bBoxMax[3]) {
     std::vector<Molecule> keep = getMolecules();
     for (int d = 0; d < 3; ++d) {
         _bbMin[d] = bBoxMin[d];
         _bbMax[d] = bBoxMax[d];
     }
-    mdx_destroy(_ctx);
-    _ctx = nullptr;
-    if (mdx_create(0, &_ctx) != MDX_OK) die("mdx_create failed");
-    check(mdx_set_components(_ctx, _model.ncomp, _model.desc.data(), (int)_model.desc.size(),
-                             _model.mix.empty() ? nullptr : _model.mix.data(), _model.epsilonRF),
-          "mdx_set_components");
-    setDomain(1);
+    makeContext(1);
     _pending = keep;
     _uploaded = false;
-    return true;
+    int32_t cpd[3];
+    double cl[3];
+    check(mdx_cell_geometry(_ctx, cpd, cl), "mdx_cell_geometry");
+    // sendParticlesTogether: false when a dimension holds fewer than two
+    // inner cells (LinkedCells.cpp:186-193)
+    return cpd[0] >= 4 && cpd[1] >= 4 && cpd[2] >= 4;
 }
 
-bool HIPLinkedCells::addParticles(std::vector<Molecule> &particles, bool) {
+void HIPLinkedCells::addParticles(std::vector<Molecule> &particles, bool) {
     if (_uploaded) {
         std::vector<Molecule> cur = getMolecules();
         _pending.insert(_pending.begin(), cur.begin(), cur.end());
         _uploaded = false;
     }
     _pending.insert(_pending.end(), particles.begin(), particles.end());
-    return true;
+}
+
+void HIPLinkedCells::setThermostat(double targetT, bool lrc, bool explosionHeuristics) {
+    _haveThermostat = true;
+    _targetT = targetT;
+    _lrc = lrc;
+    _heuristics = explosionHeuristics;
+    check(mdx_set_thermostat(_ctx, 1, &_targetT, _lrc ? 1 : 0, _heuristics ? 1 : 0),
+          "mdx_set_thermostat");
+}
+
+void HIPLinkedCells::setBoundaries(const int32_t types[3]) {
+    _haveBoundaries = true;
+    for (int d = 0; d < 3; ++d) _boundaries[d] = types[d];
+    check(mdx_set_boundaries(_ctx, _boundaries), "mdx_set_boundaries");
 }
 
 void HIPLinkedCells::update() {
@@ -148,6 +177,7 @@ std::vector<Molecule> HIPLinkedCells::getMolecules() {
 
 void HIPLinkedCells::setPrecision(int precision) {
     check(mdx_set_precision(_ctx, precision), "mdx_set_precision");
+    _precision = precision;
 }
 
 std::vector<int64_t> HIPLinkedCells::getCellIndices() {

@@ -46,7 +46,7 @@ public:
 
     bool rebuild(double bBoxMin[3], double bBoxMax[3]) override;
     void update() override;
-    bool addParticles(std::vector<Molecule> &particles, bool checkWhetherDuplicate = false) override;
+    void addParticles(std::vector<Molecule> &particles, bool checkWhetherDuplicate = false) override;
     void traverseCells(CellProcessor &cellProcessor) override;
     unsigned long getNumberOfParticles() override;
     void clear() override;
@@ -59,16 +59,28 @@ public:
 
     mdx_context *context() { return _ctx; }
     void setPrecision(int precision);
+    int getDevice() const { return _device; }
+    int getPrecision() const { return _precision; }
+    // the global velocity-scaling thermostat and the homogeneous long-range
+    // correction (mdx_set_thermostat), kept over rebuild()
+    void setThermostat(double targetT, bool lrc, bool explosionHeuristics = true);
+    // boundary type per dimension (mdx_set_boundaries), kept over rebuild()
+    void setBoundaries(const int32_t types[3]);
     // ls1 cell index of every molecule, in insertion order
     std::vector<int64_t> getCellIndices();
 
 private:
     void check(int rc, const char *what);
     void setDomain(int cellMode);
+    void makeContext(int cellMode);
     mdx_context *_ctx = nullptr;
     ComponentModel _model;
     double _bbMin[3], _bbMax[3], _cutoff, _ljCutoff;
     bool _periodic;
+    int _device, _precision;
+    bool _haveThermostat = false, _lrc = false, _heuristics = true, _haveBoundaries = false;
+    double _targetT = 0.0;
+    int32_t _boundaries[3] = {MDX_BOUNDARY_PERIODIC, MDX_BOUNDARY_PERIODIC, MDX_BOUNDARY_PERIODIC};
     std::vector<Molecule> _pending;
     std::vector<int32_t> _cids;  // component ids in upload order
     bool _uploaded = false;

@@ -115,7 +115,7 @@ public:
     virtual ~ParticleContainer() = default;
     virtual bool rebuild(double bBoxMin[3], double bBoxMax[3]) = 0;
     virtual void update() = 0;
-    virtual bool addParticles(std::vector<Molecule> &particles, bool checkWhetherDuplicate = false) = 0;
+    virtual void addParticles(std::vector<Molecule> &particles, bool checkWhetherDuplicate = false) = 0;
     virtual void traverseCells(CellProcessor &cellProcessor) = 0;
     virtual unsigned long getNumberOfParticles() = 0;
     virtual void clear() = 0;

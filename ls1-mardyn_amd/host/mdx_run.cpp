@@ -3,13 +3,18 @@
 // phase-space file, builds HIPLinkedCells + HIPCellProcessor + HIPLeapfrog
 // (what <datastructure type="HIPLinkedCells"> selects), runs
 // Simulation::prepare_start's initial force evaluation and then the step
-// order of Simulation::simulateOneTimestep (Simulation.cpp:1076-1307) without
-// thermostat, long-range correction or plugins, and prints the reference's
-// per-step line (Simulation.cpp:1383-1386).
+// order of Simulation::simulateOneTimestep (Simulation.cpp:1076-1307) with
+// the optional velocity-scaling thermostat and homogeneous long-range
+// correction but no plugins, and prints the reference's per-step line with
+// T, U_pot per molecule and the pressure (Simulation.cpp:1383-1386). As
+// Simulation::readXML does (Simulation.cpp:514), the container is rebuilt
+// right after it is built, so the cells follow LinkedCells::rebuild's float
+// cell edge.
 //
 //   mdx_run <file.inp> --cutoff RC [--lj-cutoff RCLJ] [--dt DT] [--steps N]
-//           [--precision fp64|fp32] [--open-boundary] [--checkpoint OUT.inp]
-//           [--read-write-only OUT.inp] [--nvt T] [--lrc] [--rmm]
+//           [--precision fp64|fp32] [--device N] [--open-boundary]
+//           [--checkpoint OUT.inp] [--read-write-only OUT.inp] [--nvt T] [--lrc]
+//           [--rmm]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -24,7 +29,7 @@ using namespace ls1amd;
 static void usage() {
     std::fprintf(stderr,
                  "usage: mdx_run <file.inp> --cutoff RC [--lj-cutoff RCLJ] [--dt DT] "
-                 "[--steps N] [--precision fp64|fp32] [--open-boundary] "
+                 "[--steps N] [--precision fp64|fp32] [--device N] [--open-boundary] "
                  "[--checkpoint OUT.inp] [--read-write-only OUT.inp] [--nvt T] [--lrc] "
                  "[--rmm]\n");
 }
@@ -37,7 +42,7 @@ int main(int argc, char **argv) {
     std::string inp = argv[1], checkpoint, rwonly;
     double rc = -1, rclj = -1, dt = 0.002;
     long steps = 0;
-    int precision = MDX_PREC_FP64;
+    int precision = MDX_PREC_FP64, device = 0;
     bool periodic = true, lrc = false, rmm = false;
     double Ttarget = 0.0;
     for (int i = 2; i < argc; ++i) {
@@ -54,6 +59,7 @@ int main(int argc, char **argv) {
         else if (a == "--dt") dt = std::stod(val());
         else if (a == "--steps") steps = std::stol(val());
         else if (a == "--precision") precision = val() == "fp32" ? MDX_PREC_FP32_FP64ACC : MDX_PREC_FP64;
+        else if (a == "--device") device = std::stoi(val());
         else if (a == "--open-boundary") periodic = false;
         else if (a == "--checkpoint") checkpoint = val();
         else if (a == "--read-write-only") rwonly = val();
@@ -77,15 +83,17 @@ int main(int argc, char **argv) {
         }
         if (rclj <= 0) rclj = rc;
         double bmin[3] = {0, 0, 0}, bmax[3] = {ps.box[0], ps.box[1], ps.box[2]};
-        HIPLinkedCells cells(bmin, bmax, rc, rclj, ps.model, periodic, 0, precision);
+        HIPLinkedCells cells(bmin, bmax, rc, rclj, ps.model, periodic, device, precision);
         Domain domain(ps.box);
         HIPCellProcessor proc(domain, rc, rclj);
         HIPLeapfrog integrator(dt);
         // <thermostat type="VelocityScaling"> / <longrange type="homogeneous">
-        if (Ttarget > 0 || lrc) {
-            if (mdx_set_thermostat(cells.context(), 1, &Ttarget, lrc ? 1 : 0, 1) != MDX_OK)
-                throw std::runtime_error(mdx_last_error(cells.context()));
-        }
+        if (Ttarget > 0 || lrc) cells.setThermostat(Ttarget, lrc);
+        // Simulation::readXML rebuilds the container it has just built
+        // (Simulation.cpp:514): device, precision and thermostat carry over
+        cells.rebuild(bmin, bmax);
+        if (cells.getPrecision() != precision || cells.getDevice() != device)
+            throw std::runtime_error("rebuild lost the device or the precision");
         cells.addParticles(ps.molecules);
         domain.setglobalNumMolecules(ps.molecules.size());
         // Simulation::prepare_start (Simulation.cpp:847-999)

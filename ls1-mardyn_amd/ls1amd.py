@@ -341,9 +341,10 @@ class Context:
         if cot is not None:
             c = np.ascontiguousarray(cot, np.float64).reshape(n, 3)
             out["vjp"] = np.zeros((n, 3))
-        self._chk(self.L.mdx_adjoint(self.h, n, _ptr(c), *(_ptr(out[k]) for k in (
+        fn, name = self.L.mdx_adjoint, "mdx_adjoint"
+        self._chk(fn(self.h, n, _ptr(c), *(_ptr(out[k]) for k in (
             "dU_deps", "dU_dsig", "dW_deps", "dW_dsig", "dU_dq", "dW_dq")), _ptr(out.get("vjp"))),
-            "mdx_adjoint")
+            name)
         return out
 
 

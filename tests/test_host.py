@@ -33,6 +33,19 @@ def test_phase_space_round_trip(name, tmp_path):
     assert a.eps_rf == b.eps_rf and a.ncomp == b.ncomp
 
 
+def pressure(per_step, n, volume, ndof):
+    """Domain::getGlobalPressure (Domain.cpp:142-146): p = rho T + rho W / (3 N),
+    with T = (sum m v^2 + sum I w^2) / (3 N + rotDOF) (Domain.cpp:240-242)
+    and the global virial W (local + long-range correction)."""
+    rho = n / volume
+    T = (per_step[:, 2] + per_step[:, 3]) / ndof
+    return rho * T + rho * (per_step[:, 1] / n) / 3.0
+
+
+def parsed(stdout, key):
+    return np.array([float(m) for m in re.findall(key + r" = ([-0-9.eE+]+)", stdout)])
+
+
 @pytest.mark.gpu
 def test_driver_trajectory_vs_oracle(tmp_path):
     s = scenarios.small_mixture(seed=71)
@@ -50,10 +63,39 @@ def test_driver_trajectory_vs_oracle(tmp_path):
     ref_u = np.concatenate([[sum(ref["initial"]["macro"][[0]]) / 6 + ref["initial"]["macro"][1]
                              + ref["initial"]["macro"][3]], ref["per_step"][:, 0]]) / s.n
     np.testing.assert_allclose(upot, ref_u, rtol=1e-9)
+    # the pressure of every step (NVE: no correction terms)
+    _, _, rot = o.component_props()
+    ndof = 3 * s.n + int(rot[s.cid].sum())
+    p_ref = pressure(ref["per_step"], s.n, float(np.prod(s.box)), ndof)
+    np.testing.assert_allclose(parsed(r.stdout, "p"), p_ref, rtol=1e-9)
     fin = read_inp(tmp_path / "out.inp")
     d = fin.r - ref["r"]
     d -= np.round(d / s.box) * s.box
     assert np.abs(d).max() < 1e-9
+
+
+@pytest.mark.gpu
+def test_driver_rebuild_keeps_precision(tmp_path):
+    """mdx_run rebuilds its container (as Simulation::readXML does) after
+    setting the FP32 form: the run stays in the FP32 form (its U_pot differs
+    from the FP64 run's, within the form's error bound) and on the device it
+    was given; the FP64 run matches the oracle."""
+    s = scenarios.small_lj(seed=74)
+    f = tmp_path / "lj.inp"
+    write_inp(f, s.box, s.ncomp, s.comp_desc, s.mix, s.eps_rf, np.arange(1, s.n + 1), s.cid, s.r,
+              s.v, s.q, s.L)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        r = subprocess.run([str(MDX_RUN), str(f), "--cutoff", str(s.rc), "--dt", "0.002",
+                            "--steps", "5", "--precision", prec, "--device", "0"],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        out[prec] = parsed(r.stdout, "U_pot")
+    o = Oracle(1, s.comp_desc, None, 1.0, s.rc, s.rc, s.box, periodic=True, cell_mode=1)
+    ref = o.simulate(5, 0.002, s.r, s.v, s.q, s.L, s.cid)
+    np.testing.assert_allclose(out["fp64"][1:], ref["per_step"][:, 0] / s.n, rtol=1e-9)
+    rel = np.abs(out["fp32"] - out["fp64"]) / np.abs(out["fp64"])
+    assert rel.max() < 1e-5 and rel.max() > 1e-12
 
 
 @pytest.mark.gpu
@@ -75,6 +117,11 @@ def test_driver_nvt_lrc_vs_oracle(tmp_path):
     ref = o.simulate_nvt(20, 0.002, s.r, s.v, s.q, s.L, s.cid, 0.9, lrc=True)
     np.testing.assert_allclose(upot, ref["per_step"][:, 0] / s.n, rtol=1e-9)
     np.testing.assert_allclose(temp, ref["per_step"][:, 6], rtol=1e-9)
+    # p from the step's T (before the velocity scaling) and the global virial
+    # including the long-range correction
+    rho = s.n / float(np.prod(s.box))
+    p_ref = rho * ref["per_step"][:, 6] + rho * (ref["per_step"][:, 1] / s.n) / 3.0
+    np.testing.assert_allclose(parsed(r.stdout, "p"), p_ref, rtol=1e-9)
     fin = read_inp(tmp_path / "out.inp")
     d = fin.r - ref["r"]
     d -= np.round(d / s.box) * s.box
