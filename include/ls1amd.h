@@ -145,14 +145,22 @@ int mdx_set_thermostat(mdx_context *ctx, int nrep, const double *T_target, int l
 
 /* Boundary type per dimension, replacing the <boundaries> section
  * (Simulation.cpp:429-444) after mdx_set_domain / mdx_set_replicas:
- * MDX_BOUNDARY_PERIODIC (images across the box, wrap), or
- * MDX_BOUNDARY_REFLECTING: no images in that dimension (the reference deletes
- * those halos, BoundaryHandler::removeNonPeriodicHalos) and, before every
- * upd_preF, BoundaryHandler::processGlobalWallLeavingParticles
- * (BoundaryHandler.cpp:82-134) reverses the velocity of a molecule within rc
- * of the wall that would cross it. MDX_BOUNDARY_OUTFLOW (deleting leaving
- * molecules) is rejected: the molecule count is fixed per upload. */
+ * MDX_BOUNDARY_PERIODIC (images across the box, wrap), or a wall with no
+ * images in that dimension (the reference deletes those halos,
+ * BoundaryHandler::removeNonPeriodicHalos, BoundaryHandler.cpp:143-170) where,
+ * before every upd_preF, BoundaryHandler::processGlobalWallLeavingParticles
+ * (BoundaryHandler.cpp:82-134) handles a molecule within rc of the wall that
+ * would cross it: MDX_BOUNDARY_REFLECTING reverses its velocity,
+ * MDX_BOUNDARY_OUTFLOW deletes it. Outflow walls serve single systems without
+ * the thermostat, the long-range correction or the reduced-memory mode (they
+ * keep the molecule count fixed); with them mdx_download and mdx_cell_ids
+ * return the surviving molecules in upload order (ids tell which), and
+ * mdx_molecule_count gives their number. */
 int mdx_set_boundaries(mdx_context *ctx, const int32_t *types);
+
+/* Molecules in the context: the uploaded count, less those deleted at outflow
+ * walls (ParticleContainer::getNumberOfParticles, ParticleContainer.h:160). */
+int mdx_molecule_count(mdx_context *ctx, int64_t *n);
 
 /* Reduced-memory one-centre mode: the VCP1CLJRMM processor with LeapfrogRMM
  * (VCP1CLJRMM.cpp:300-362, LeapfrogRMM.cpp:42-107, PositionCellProcessorRMM.h,
@@ -251,13 +259,26 @@ int mdx_cell_pairs(mdx_context *ctx, int32_t *pairs, int64_t max_pairs, int64_t 
 int mdx_set_counting(mdx_context *ctx, int enable);
 int mdx_counters(mdx_context *ctx, int64_t *out);
 
+/* Per pair kernel (one per i-site type: LJ, charge, dipole, quadrupole), for
+ * the roofline of each: ms[4] = its duration in the last traversal timed with
+ * mdx_set_timing (HIP events around each launch; 0 for an absent type);
+ * counters[4][8] = its raw visit counts in the last counted traversal, in the
+ * mdx_counters categories. Every site evaluates its own side of a pair, so an
+ * unordered pair of two types is visited once by each type's kernel and a
+ * pair of one type twice by its kernel; mdx_counters is half the sum over the
+ * kernels. Either pointer may be NULL. No counterpart in the reference (its
+ * FlopCounter counts one processor over all types, FlopCounter.cpp:19-76). */
+int mdx_kernel_stats(mdx_context *ctx, double *ms, int64_t *counters);
+
 /* Device-resident variants for callers that keep the state in HBM: device
  * pointers to n doubles / ints laid out as in mdx_upload. */
 int mdx_upload_device(mdx_context *ctx, int64_t n, const double *r, const double *v,
                       const double *q, const double *L, const int32_t *cid);
 
 /* Time of the last traversal's pair kernels in milliseconds, measured with
- * HIP events on the context's stream (0 if not measured). */
+ * HIP events on the context's stream (0 if not measured); after an adjoint
+ * call with timing on: pair_ms = its derivative kernels, total_ms = up to the
+ * assembled VJP (host transfers excluded). */
 int mdx_set_timing(mdx_context *ctx, int enable);
 int mdx_last_kernel_ms(mdx_context *ctx, double *pair_ms, double *total_ms);
 

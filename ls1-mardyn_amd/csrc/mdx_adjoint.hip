@@ -10,21 +10,38 @@
 //  * the force vector-Jacobian product vjp_i = sum_j dF_j/dr_i . c_j with
 //    respect to molecule positions (orientations held fixed), for a
 //    cotangent c on the molecule forces.
-// Layout: one 64-lane block walks a strided set of cells; lane = (i-site,
-// j-phase) as in the forward kernels; j sites are read straight from the
-// coalesced SoA site arrays. Every site accumulates only its own terms
-// (owner computes), so each unordered pair is visited from both sides: the
-// parameter derivatives are halved per visit and summed over both index
-// orders on the host. Parameter sums go through per-lane LDS columns and a
-// fixed-order per-block fold, then a fixed-order reduction over blocks, so
+// Layout, as the forward pair kernels: one 64-lane block walks a strided set
+// of cells; the cell's i sites occupy nI lanes per phase and P = min(64 / nI,
+// 8) phases split the partners; the 27 neighbour images are staged into LDS
+// in chunks (site position and molecule position with the image shift
+// applied, the partner molecule's cotangent), a stage-A pass keeps per-lane
+// hit lists of molecule-centre distances below the cutoff and stage B
+// evaluates the derivative terms of the hits. Every site accumulates only its
+// own terms (owner computes), so each unordered pair is visited from both
+// sides: the parameter derivatives are halved per visit and summed over both
+// index orders on the host. Parameter sums go through per-lane LDS columns and
+// a fixed-order per-block fold, then a fixed-order reduction over blocks, so
 // results are deterministic.
 #include "mdx_adjoint.h"
 #include "mdx_pair_common.h"
 
 namespace mdx {
 
+constexpr int kAdjCap = 128;  // partner sites per LDS chunk
+constexpr int kAdjList = 16;  // per-lane hit list entries
+
+struct AdjTile {
+    Q4<double> p[kAdjCap];  // site position, i cell frame; w = LJ id or charge
+    Q4<double> m[kAdjCap];  // molecule position, i cell frame
+    Q4<double> c[kAdjCap];  // cotangent on the partner molecule's force
+    unsigned short lst[kAdjList][64];
+    int j0[27], pre[28];
+    double shift[27][3];
+};
+
 template <int TI>
 __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
+    __shared__ AdjTile T;
     extern __shared__ double lds[];
     // per-lane columns [W][64], the block table [K*W], lane keys [64]
     const int W = a.W, K = a.K, TW = K * W;
@@ -37,23 +54,49 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
     const SiteArrays &S = a.s;
     const double rc2 = TI == T_LJ ? a.g.rclj2 : a.g.rc2;
     const int nl = a.nlj;
+    const bool vjp = a.cx != nullptr;
     for (int cell = blockIdx.x; cell < a.ncell; cell += gridDim.x) {
         const int i_begin = S.cstart[cell], i_end = S.cstart[cell + 1];
+        if (i_begin == i_end) continue;
         const int cx = cell % a.g.n[0], cy = (cell / a.g.n[0]) % a.g.n[1],
                   cz = cell / (a.g.n[0] * a.g.n[1]);
+        // the 27 neighbour ranges and their prefix (same for every chunk)
+        int nk = 0;
+        if (lane < 27) {
+            Nbr b = neighbour(a.g, cx, cy, cz, lane);
+            int j0 = 0;
+            if (b.ok) {
+                j0 = S.cstart[b.cell];
+                nk = S.cstart[b.cell + 1] - j0;
+            }
+            T.j0[lane] = j0;
+            T.shift[lane][0] = b.s[0];
+            T.shift[lane][1] = b.s[1];
+            T.shift[lane][2] = b.s[2];
+        }
+        int incl = nk;
+        for (int o = 1; o < 32; o <<= 1) {
+            int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (lane < 27) T.pre[lane + 1] = incl;
+        if (lane == 0) T.pre[0] = 0;
+        const int total = __shfl(incl, 26, 64);
+        __syncthreads();
         for (int chunk = i_begin; chunk < i_end; chunk += 64) {
             const int nI = min(64, i_end - chunk);
-            int lg = 0;
-            while ((1 << lg) < nI) ++lg;
-            const int Ipad = 1 << lg, P = 64 >> lg;
-            const int il = lane & (Ipad - 1), ph = lane >> lg;
-            const bool valid = il < nI;
+            const int P = min(64 / nI, 8);
+            const int ph0 = lane / nI;
+            const int il = lane - ph0 * nI;
+            const bool valid = ph0 < P;
+            const int ph = valid ? ph0 : 0;
             const int i = chunk + (valid ? il : 0);
             const double px = S.x[i], py = S.y[i], pz = S.z[i];
-            const double mx = S.mx[i], my = S.my[i], mz = S.mz[i];
+            // idle lanes test from far away and never hit
+            const double mx = valid ? S.mx[i] : 1e300, my = S.my[i], mz = S.mz[i];
             const int mi = S.mol[i];
             double ci[3] = {0, 0, 0};
-            if (a.cx) ci[0] = a.cx[mi], ci[1] = a.cy[mi], ci[2] = a.cz[mi];
+            if (vjp) ci[0] = a.cx[mi], ci[1] = a.cy[mi], ci[2] = a.cz[mi];
             int key;
             double qi = 0.0;
             if constexpr (TI == T_LJ) {
@@ -63,65 +106,104 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                 qi = S.p[i];
             }
             double g[3] = {0, 0, 0};
-            for (int k = 0; k < 27; ++k) {
-                Nbr b = neighbour(a.g, cx, cy, cz, k);
-                if (!b.ok) continue;
-                const int j0 = S.cstart[b.cell], j1 = S.cstart[b.cell + 1];
-                for (int j = j0 + ph; j < j1; j += P) {
-                    if (!valid) break;
-                    const double dmx = mx - (S.mx[j] + b.s[0]), dmy = my - (S.my[j] + b.s[1]),
-                                 dmz = mz - (S.mz[j] + b.s[2]);
-                    const double r2m = dmx * dmx + dmy * dmy + dmz * dmz;
-                    if (!(r2m < rc2)) continue;
-                    if (k == 13 && r2m == 0.0) continue;
-                    const double dx = px - (S.x[j] + b.s[0]), dy = py - (S.y[j] + b.s[1]),
-                                 dz = pz - (S.z[j] + b.s[2]);
-                    const double r2 = dx * dx + dy * dy + dz * dz, r2i = 1.0 / r2;
-                    const double dmd = dmx * dx + dmy * dy + dmz * dz;
-                    const int mj = S.mol[j];
-                    double dc[3] = {0, 0, 0};
-                    if (a.cx) dc[0] = ci[0] - a.cx[mj], dc[1] = ci[1] - a.cy[mj], dc[2] = ci[2] - a.cz[mj];
-                    const double ddc = dx * dc[0] + dy * dc[1] + dz * dc[2];
-                    double sc, sp;
-                    if constexpr (TI == T_LJ) {
-                        const int bj = S.ljid[j], kk = key * nl + bj;
-                        const double e24 = a.pp.eps24[kk], s2 = a.pp.sig2[kk];
-                        const double eps = e24 / 24.0, sig = sqrt(s2);
-                        double l6 = s2 * r2i;
-                        l6 = l6 * l6 * l6;
-                        const double l12 = l6 * l6;
-                        sc = e24 * r2i * (2.0 * l12 - l6);
-                        sp = e24 * r2i * r2i * (4.0 * l6 - 14.0 * l12);
-                        double gUe = 4.0 * (l12 - l6), gUs = 4.0 * eps * (12.0 * l12 - 6.0 * l6) / sig;
-                        const double rcs = a.shift_rc[kk];
-                        if (rcs > 0.0) {
-                            double s6 = s2 / (rcs * rcs);
-                            s6 = s6 * s6 * s6;
-                            gUe += 4.0 * (s6 - s6 * s6);
-                            gUs += 4.0 * eps * (6.0 * s6 - 12.0 * s6 * s6) / sig;
-                        }
-                        const double gWe = dmd * 24.0 * r2i * (2.0 * l12 - l6);
-                        const double gWs = dmd * e24 * r2i * (24.0 * l12 - 6.0 * l6) / sig;
-                        double *c0 = col + (size_t)(bj * 4) * 64 + lane;
-                        c0[0] += 0.5 * gUe;
-                        c0[64] += 0.5 * gUs;
-                        c0[128] += 0.5 * gWe;
-                        c0[192] += 0.5 * gWs;
-                    } else {
-                        const double qj = S.p[j];
-                        const double ri = sqrt(r2i), r3i = r2i * ri;
-                        col[lane] += qj * ri;
-                        col[64 + lane] += qj * dmd * r3i;
-                        sc = qi * qj * r3i;
-                        sp = -1.5 * qi * qj * r3i * r2i;
+            for (int base = 0; base < total; base += kAdjCap) {
+                const int nst = min(kAdjCap, total - base);
+                for (int t = lane; t < nst; t += 64) {
+                    const int idx = base + t;
+                    const int k = tile_k(T.pre, idx);
+                    const int j = T.j0[k] + (idx - T.pre[k]);
+                    const double sx = T.shift[k][0], sy = T.shift[k][1], sz = T.shift[k][2];
+                    const double w = TI == T_LJ ? (double)S.ljid[j] : S.p[j];
+                    T.p[t] = Q4<double>{S.x[j] + sx, S.y[j] + sy, S.z[j] + sz, w};
+                    T.m[t] = Q4<double>{S.mx[j] + sx, S.my[j] + sy, S.mz[j] + sz, 0.0};
+                    if (vjp) {
+                        const int mj = S.mol[j];
+                        T.c[t] = Q4<double>{a.cx[mj], a.cy[mj], a.cz[mj], 0.0};
                     }
-                    g[0] += sc * dc[0] + 2.0 * sp * dx * ddc;
-                    g[1] += sc * dc[1] + 2.0 * sp * dy * ddc;
-                    g[2] += sc * dc[2] + 2.0 * sp * dz * ddc;
                 }
+                __syncthreads();
+                int cnt = 0;
+                // stage B: the derivative terms of the listed partners
+                auto drain = [&]() {
+                    for (int e = 0; __any(e < cnt); ++e) {
+                        if (e >= cnt) continue;
+                        const int t = T.lst[e][lane];
+                        const Q4<double> pj = T.p[t], mj4 = T.m[t];
+                        const double dmx = mx - mj4.x, dmy = my - mj4.y, dmz = mz - mj4.z;
+                        const double dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
+                        const double r2 = dx * dx + dy * dy + dz * dz, r2i = 1.0 / r2;
+                        const double dmd = dmx * dx + dmy * dy + dmz * dz;
+                        double dc[3] = {0, 0, 0};
+                        if (vjp) {
+                            const Q4<double> cj = T.c[t];
+                            dc[0] = ci[0] - cj.x, dc[1] = ci[1] - cj.y, dc[2] = ci[2] - cj.z;
+                        }
+                        const double ddc = dx * dc[0] + dy * dc[1] + dz * dc[2];
+                        double sc, sp;
+                        if constexpr (TI == T_LJ) {
+                            const int bj = (int)pj.w, kk = key * nl + bj;
+                            const double e24 = a.pp.eps24[kk], s2 = a.pp.sig2[kk];
+                            const double eps = e24 / 24.0, sig = sqrt(s2);
+                            double l6 = s2 * r2i;
+                            l6 = l6 * l6 * l6;
+                            const double l12 = l6 * l6;
+                            sc = e24 * r2i * (2.0 * l12 - l6);
+                            sp = e24 * r2i * r2i * (4.0 * l6 - 14.0 * l12);
+                            double gUe = 4.0 * (l12 - l6), gUs = 4.0 * eps * (12.0 * l12 - 6.0 * l6) / sig;
+                            const double rcs = a.shift_rc[kk];
+                            if (rcs > 0.0) {
+                                double s6 = s2 / (rcs * rcs);
+                                s6 = s6 * s6 * s6;
+                                gUe += 4.0 * (s6 - s6 * s6);
+                                gUs += 4.0 * eps * (6.0 * s6 - 12.0 * s6 * s6) / sig;
+                            }
+                            const double gWe = dmd * 24.0 * r2i * (2.0 * l12 - l6);
+                            const double gWs = dmd * e24 * r2i * (24.0 * l12 - 6.0 * l6) / sig;
+                            double *c0 = col + (size_t)(bj * 4) * 64 + lane;
+                            c0[0] += 0.5 * gUe;
+                            c0[64] += 0.5 * gUs;
+                            c0[128] += 0.5 * gWe;
+                            c0[192] += 0.5 * gWs;
+                        } else {
+                            const double qj = pj.w;
+                            const double ri = sqrt(r2i), r3i = r2i * ri;
+                            col[lane] += qj * ri;
+                            col[64 + lane] += qj * dmd * r3i;
+                            sc = qi * qj * r3i;
+                            sp = -1.5 * qi * qj * r3i * r2i;
+                        }
+                        g[0] += sc * dc[0] + 2.0 * sp * dx * ddc;
+                        g[1] += sc * dc[1] + 2.0 * sp * dy * ddc;
+                        g[2] += sc * dc[2] + 2.0 * sp * dz * ddc;
+                    }
+                    cnt = 0;
+                };
+                // stage A: molecule-centre cutoff, strictly less; the molecule
+                // itself (distance 0) is skipped (SingleCellPolicy_)
+                for (int m = ph; m - ph < nst; m += P) {
+                    if (__any(cnt >= kAdjList)) drain();
+                    bool hit = false;
+                    if (m < nst) {
+                        const Q4<double> q = T.m[m];
+                        const double ex = mx - q.x, ey = my - q.y, ez = mz - q.z;
+                        const double r2m = ex * ex + ey * ey + ez * ez;
+                        hit = r2m < rc2 && r2m != 0.0;
+                    }
+                    T.lst[cnt][lane] = (unsigned short)m;
+                    cnt += hit ? 1 : 0;
+                }
+                drain();
+                __syncthreads();
             }
-            for (int o = Ipad; o < 64; o <<= 1)
-                for (int d = 0; d < 3; ++d) g[d] += __shfl_xor(g[d], o, 64);
+            // the phases of each site into its phase-0 lane, in phase order
+            // (every shuffle reads the lanes' own partial sums)
+            const double own[3] = {g[0], g[1], g[2]};
+            for (int q = 1; q < 8; ++q) {
+                const int src = min(il + q * nI, 63);
+                const double t0 = __shfl(own[0], src, 64), t1 = __shfl(own[1], src, 64),
+                             t2 = __shfl(own[2], src, 64);
+                if (q < P) g[0] += t0, g[1] += t1, g[2] += t2;
+            }
             if (valid && ph == 0 && a.gx) {
                 a.gx[i] = g[0];
                 a.gy[i] = g[1];

@@ -18,7 +18,7 @@ __global__ void k_leapfrog_pre(MolArrays, const double *, const double *, const 
 __global__ void k_leapfrog_post_pre(MolArrays, const double *, const double *, const Component *,
                                     Reps, double, int, const int *, const int *, double *);
 __global__ void k_leapfrog_post(MolArrays, const double *, const double *, const Component *,
-                                double, int, const int *, const int *, double *, int);
+                                double, int, const int *, const int *, double *, int, int);
 __global__ void k_seg_reduce(const double *, int, const int *, double *);
 __global__ void k_reduce_partials(const double *, int, int, int, double *);
 __global__ void k_fill_int(int *, int, int);
@@ -138,6 +138,11 @@ struct mdx_context {
     std::vector<uint64_t> ids;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double pair_ms = 0, total_ms = 0;
+    // per i-site type pair kernel: an event before each launch and one after
+    // the last, and the last timed traversal's times
+    hipEvent_t kev[kTypes + 1] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int kev_order[kTypes] = {-1, -1, -1, -1};
+    double kernel_ms[kTypes] = {0, 0, 0, 0};
     // captured steps per kind (full step, step fused with the next one's
     // first half, last step of a fused run) and state-buffer parity
     hipGraphExec_t gexec[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
@@ -723,11 +728,13 @@ int launch_update(mdx_context *c) {
     const int n = c->n;
     if (c->point_lj) ++c->q_renorm;
     MolArrays &src = c->mol[c->cur], &dst = c->mol[1 - c->cur];
-    hipLaunchKernelGGL(k_fill_int, dim3(nblk(c->ncell + 1)), dim3(kBlock), 0, c->stream, c->count,
-                       c->ncell + 1, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(c->ncell + 2)), dim3(kBlock), 0, c->stream, c->count,
+                       c->ncell + 2, 0);
     hipLaunchKernelGGL(k_bin_count, dim3(nblk(n)), dim3(kBlock), 0, c->stream, src, reps(c), n,
                        c->cell, c->rank, c->count, c->ls1id);
-    TRY(scan_exclusive(c, c->count, c->start, c->ncell));
+    // the cells and the bin of deleted molecules behind them: start[ncell] is
+    // the live count, start[ncell + 1] = n
+    TRY(scan_exclusive(c, c->count, c->start, c->ncell + 1));
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, c->cell,
                        c->rank, c->start, c->perm);
     if (c->point_lj) {
@@ -798,7 +805,6 @@ int launch_traverse(mdx_context *c, double kick = 0.0) {
     a.ncell = c->ncell;
     a.cpb = c->cpb;
     int grid = pair_grid(c, &a.items_per_xcd);
-    a.counters = c->d_counters;
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     // one [ncell][4] partial array: the first i-site type writes each cell's
     // row, later types add to it (the launches are ordered on the stream)
@@ -807,10 +813,19 @@ int launch_traverse(mdx_context *c, double kick = 0.0) {
         if (c->nsites[t] == 0) continue;
         a.macro_part = c->macro_part;
         a.accumulate = slot > 0;
+        a.counters = c->d_counters + 8 * t;  // each kernel its own row
         PairKernel k = pair_kernel(t, c->precision == MDX_PREC_FP32_FP64ACC, c->counting,
                                    t == T_LJ && c->lj_at_origin, c->model.nlj > 1);
+        if (c->timing) {
+            HIPCHK(c, hipEventRecord(c->kev[slot], c->stream));
+            c->kev_order[slot] = t;
+        }
         hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, c->stream, a);
         ++slot;
+    }
+    if (c->timing) {
+        HIPCHK(c, hipEventRecord(c->kev[slot], c->stream));
+        for (int k = slot; k < kTypes; ++k) c->kev_order[k] = -1;
     }
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     const int R = c->nrep;
@@ -834,6 +849,12 @@ int launch_traverse(mdx_context *c, double kick = 0.0) {
     return MDX_OK;
 }
 
+// The domain has an outflow wall: molecules can be deleted, the live count
+// can drop below the uploaded one.
+bool open_domain(const mdx_context *c) {
+    return c->g.outflow[0] | c->g.outflow[1] | c->g.outflow[2];
+}
+
 int launch_lf_pre(mdx_context *c, double dt) {
     // reduced-memory mode: PositionCellProcessorRMM, r += dt v only
     hipLaunchKernelGGL(k_leapfrog_pre, dim3(nblk(c->n)), dim3(kBlock), 0, c->stream,
@@ -847,7 +868,7 @@ int launch_lf_post(mdx_context *c, double dt) {
     const int R = c->nrep, nb = c->lf_nchunk;
     hipLaunchKernelGGL(k_leapfrog_post, dim3(nb), dim3(kBlock), 0, c->stream, c->mol[c->cur], c->F,
                        c->M, c->d_comps, dt, c->n, c->d_lf_chunk, c->d_lf_chunk_end, c->lf_part,
-                       c->rmm_dt != 0.0 ? 1 : 0);
+                       c->rmm_dt != 0.0 ? 1 : 0, open_domain(c) ? 1 : 0);
     if (R == 1)
         launch_reduce(c, c->lf_part, nb, 4, c->red + 8);
     else
@@ -956,8 +977,10 @@ int state_setup(mdx_context *c, int64_t n64, const int32_t *cid_host) {
     TRY(dalloc(c, &c->mcell, n, true));
     TRY(dalloc(c, &c->ls1id, n, true));
     TRY(dalloc(c, &c->mls1id, n, true));
-    TRY(dalloc(c, &c->count, c->ncell + 1, true));
-    TRY(dalloc(c, &c->start, c->ncell + 1, true));
+    // one bin per cell, one behind the last cell for molecules deleted at an
+    // outflow wall, and the scan total
+    TRY(dalloc(c, &c->count, c->ncell + 2, true));
+    TRY(dalloc(c, &c->start, c->ncell + 2, true));
     for (int t = 0; t < kTypes; ++t) {
         c->nsite_mol[t] = nullptr;
         if (!c->point_lj) TRY(dalloc(c, &c->nsite_mol[t], n + 1, true));
@@ -1056,7 +1079,9 @@ int mdx_create(int device, mdx_context **out) {
         return MDX_ERR_HIP;
     }
     for (auto &e : c->ev) (void)hipEventCreate(&e);
-    if (dalloc(c, &c->red_tmp, 256 * 8) || dalloc(c, &c->d_counters, 8) || dalloc(c, &c->step_idx, 1) ||
+    for (auto &e : c->kev) (void)hipEventCreate(&e);
+    if (dalloc(c, &c->red_tmp, 256 * 8) || dalloc(c, &c->d_counters, 8 * kTypes) ||
+        dalloc(c, &c->step_idx, 1) ||
         dalloc(c, &c->d_comps, kMaxComp) || ensure_steps(c, 1)) {
         g_create_error = c->err;
         free_list(c->allocs);
@@ -1064,7 +1089,7 @@ int mdx_create(int device, mdx_context **out) {
         return MDX_ERR_HIP;
     }
     (void)hipMemset(c->step_idx, 0, sizeof(int));
-    (void)hipMemset(c->d_counters, 0, 8 * sizeof(unsigned long long));
+    (void)hipMemset(c->d_counters, 0, 8 * kTypes * sizeof(unsigned long long));
     *out = c;
     return MDX_OK;
 }
@@ -1077,6 +1102,8 @@ int mdx_destroy(mdx_context *c) {
     free_list(c->state_allocs);
     free_list(c->allocs);
     for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->kev)
         if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1196,6 +1223,11 @@ int mdx_set_thermostat(mdx_context *c, int nrep, const double *T_target, int lrc
     if (!c) return MDX_ERR_ARG;
     TRY(need(c, true, true, false));
     if (nrep != c->nrep) return fail(c, MDX_ERR_ARG, "nrep differs from the replica count");
+    const bool any_t = T_target && std::any_of(T_target, T_target + nrep, [](double t) { return t > 0; });
+    if (open_domain(c) && (any_t || lrc))
+        return fail(c, MDX_ERR_ARG,
+                    "the thermostat and the long-range correction keep the molecule count fixed; "
+                    "the domain has outflow walls");
     HIPCHK(c, hipSetDevice(c->device));
     c->T_target.assign((size_t)nrep, 0.0);
     c->thermostat = false;
@@ -1213,25 +1245,34 @@ int mdx_set_thermostat(mdx_context *c, int nrep, const double *T_target, int lrc
 int mdx_set_boundaries(mdx_context *c, const int32_t *types) {
     if (!c || !types) return MDX_ERR_ARG;
     TRY(need(c, true, true, false));
+    bool outflow = false;
     for (int d = 0; d < 3; ++d) {
-        if (types[d] == MDX_BOUNDARY_OUTFLOW)
-            return fail(c, MDX_ERR_ARG, "outflow boundaries (molecule deletion) are not supported");
-        if (types[d] != MDX_BOUNDARY_PERIODIC && types[d] != MDX_BOUNDARY_REFLECTING)
+        if (types[d] != MDX_BOUNDARY_PERIODIC && types[d] != MDX_BOUNDARY_REFLECTING &&
+            types[d] != MDX_BOUNDARY_OUTFLOW)
             return fail(c, MDX_ERR_ARG, "unknown boundary type");
+        outflow = outflow || types[d] == MDX_BOUNDARY_OUTFLOW;
     }
     if (c->rmm_dt != 0.0)
-        return fail(c, MDX_ERR_STATE, "reflecting walls need the stored forces (not in reduced-memory mode)");
+        return fail(c, MDX_ERR_STATE, "walls need the stored forces (not in reduced-memory mode)");
+    if (outflow && c->nrep > 1)
+        return fail(c, MDX_ERR_ARG, "outflow walls serve single systems (not batched replicas)");
+    if (outflow && (c->thermostat || c->lrc))
+        return fail(c, MDX_ERR_ARG,
+                    "outflow walls change the molecule count; the thermostat and the long-range "
+                    "correction keep it fixed");
     HIPCHK(c, hipSetDevice(c->device));
     for (Geom *g : {&c->g}) {
         for (int d = 0; d < 3; ++d) {
             g->pdim[d] = types[d] == MDX_BOUNDARY_PERIODIC;
             g->reflect[d] = types[d] == MDX_BOUNDARY_REFLECTING;
+            g->outflow[d] = types[d] == MDX_BOUNDARY_OUTFLOW;
         }
     }
     for (Geom &g : c->geoms)
         for (int d = 0; d < 3; ++d) {
             g.pdim[d] = c->g.pdim[d];
             g.reflect[d] = c->g.reflect[d];
+            g.outflow[d] = c->g.outflow[d];
         }
     HIPCHK(c, hipMemcpy(c->d_geoms, c->geoms.data(), sizeof(Geom) * c->nrep, hipMemcpyHostToDevice));
     drop_graphs(c);
@@ -1250,8 +1291,8 @@ int mdx_set_rmm(mdx_context *c, double dt) {
     if (c->model.ncomp != 1 || !c->point_lj)
         return fail(c, MDX_ERR_ARG, "reduced-memory mode: one component of one LJ centre only");
     for (int d = 0; d < 3; ++d)
-        if (c->g.reflect[d])
-            return fail(c, MDX_ERR_STATE, "reduced-memory mode stores no forces for reflecting walls");
+        if (c->g.reflect[d] || c->g.outflow[d])
+            return fail(c, MDX_ERR_STATE, "reduced-memory mode stores no forces for the wall rule");
     if (c->rmm_dt != 0.0) return fail(c, MDX_ERR_STATE, "reduced-memory mode is already on");
     HIPCHK(c, hipSetDevice(c->device));
     drop_graphs(c);
@@ -1417,11 +1458,13 @@ int mdx_upload_device(mdx_context *c, int64_t n, const double *r, const double *
     return MDX_OK;
 }
 
-int mdx_download(mdx_context *c, int64_t n, double *r, double *v, double *q, double *L, double *F,
+}  // extern "C"
+
+namespace {
+
+// Every uploaded molecule (live or deleted) in upload order.
+int download_all(mdx_context *c, double *r, double *v, double *q, double *L, double *F,
                  double *M, double *Vi, uint64_t *ids) {
-    if (!c) return MDX_ERR_ARG;
-    TRY(need(c, true, true, true));
-    if (n != c->n) return fail(c, MDX_ERR_ARG, "n differs from the uploaded count");
     HIPCHK(c, hipSetDevice(c->device));
     const int nn = c->n;
     MolArrays &m = c->mol[c->cur];
@@ -1483,6 +1526,81 @@ int mdx_download(mdx_context *c, int64_t n, double *r, double *v, double *q, dou
     return MDX_OK;
 }
 
+// Upload-order flags of the molecules not deleted at an outflow wall (a
+// deleted molecule's position is NaN); returns their count.
+int64_t live_mask(mdx_context *c, std::vector<char> &alive, int *rc) {
+    std::vector<double> r(3 * (size_t)c->n);
+    *rc = download_all(c, r.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    alive.assign((size_t)c->n, 1);
+    int64_t k = 0;
+    for (int i = 0; i < c->n; ++i) {
+        alive[i] = r[3 * (size_t)i] == r[3 * (size_t)i];
+        k += alive[i];
+    }
+    return k;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdx_download(mdx_context *c, int64_t n, double *r, double *v, double *q, double *L, double *F,
+                 double *M, double *Vi, uint64_t *ids) {
+    if (!c) return MDX_ERR_ARG;
+    TRY(need(c, true, true, true));
+    if (!open_domain(c)) {
+        if (n != c->n) return fail(c, MDX_ERR_ARG, "n differs from the uploaded count");
+        return download_all(c, r, v, q, L, F, M, Vi, ids);
+    }
+    // outflow walls: the survivors, in upload order
+    const size_t N = c->n;
+    std::vector<double> rr(3 * N), vv(v ? 3 * N : 0), qq(q ? 4 * N : 0), LL(L ? 3 * N : 0),
+        FF(F ? 3 * N : 0), MM(M ? 3 * N : 0), VV(Vi ? 3 * N : 0);
+    std::vector<uint64_t> ii(ids ? N : 0);
+    auto dp = [](std::vector<double> &a) { return a.empty() ? nullptr : a.data(); };
+    TRY(download_all(c, rr.data(), dp(vv), dp(qq), dp(LL), dp(FF), dp(MM), dp(VV),
+                     ids ? ii.data() : nullptr));
+    int64_t live = 0;
+    for (size_t i = 0; i < N; ++i) live += rr[3 * i] == rr[3 * i];
+    if (n != live)
+        return fail(c, MDX_ERR_ARG,
+                    "n differs from the live molecule count " + std::to_string(live) +
+                        " (mdx_molecule_count)");
+    size_t k = 0;
+    for (size_t i = 0; i < N; ++i) {
+        if (rr[3 * i] != rr[3 * i]) continue;
+        auto put = [&](double *out, const std::vector<double> &a, int w) {
+            if (out)
+                for (int d = 0; d < w; ++d) out[k * w + d] = a[i * w + d];
+        };
+        put(r, rr, 3);
+        put(v, vv, 3);
+        put(q, qq, 4);
+        put(L, LL, 3);
+        put(F, FF, 3);
+        put(M, MM, 3);
+        put(Vi, VV, 3);
+        if (ids) ids[k] = ii[i];
+        ++k;
+    }
+    return MDX_OK;
+}
+
+int mdx_molecule_count(mdx_context *c, int64_t *n) {
+    if (!c || !n) return MDX_ERR_ARG;
+    TRY(need(c, true, true, true));
+    if (!open_domain(c)) {
+        *n = c->n;
+        return MDX_OK;
+    }
+    std::vector<char> alive;
+    int rc = MDX_OK;
+    const int64_t k = live_mask(c, alive, &rc);
+    TRY(rc);
+    *n = k;
+    return MDX_OK;
+}
+
 int mdx_update_container(mdx_context *c) {
     if (!c) return MDX_ERR_ARG;
     TRY(need(c, true, true, true));
@@ -1498,7 +1616,8 @@ int mdx_traverse(mdx_context *c, double *macro) {
     HIPCHK(c, hipSetDevice(c->device));
     TRY(ensure_binned(c));
     if (c->counting)
-        HIPCHK(c, hipMemsetAsync(c->d_counters, 0, 8 * sizeof(unsigned long long), c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_counters, 0, 8 * kTypes * sizeof(unsigned long long),
+                                 c->stream));
     HIPCHK(c, hipMemsetAsync(c->step_idx, 0, sizeof(int), c->stream));
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     // in the reduced-memory mode the traversal is VCP1CLJRMM's: force + kick
@@ -1516,6 +1635,12 @@ int mdx_traverse(mdx_context *c, double *macro) {
         HIPCHK(c, hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
         c->pair_ms = a;
         c->total_ms = b;
+        for (int t = 0; t < kTypes; ++t) c->kernel_ms[t] = 0.0;
+        for (int k = 0; k < kTypes && c->kev_order[k] >= 0; ++k) {
+            float e = 0;
+            HIPCHK(c, hipEventElapsedTime(&e, c->kev[k], c->kev[k + 1]));
+            c->kernel_ms[c->kev_order[k]] = e;
+        }
     }
     return MDX_OK;
 }
@@ -1554,7 +1679,7 @@ int mdx_step(mdx_context *c, int nsteps, double dt, double *per_step, int use_gr
     // Consecutive steps without a thermostat between them share one pass
     // over the molecules for step n's second and step n + 1's first half
     // kick (NVE; the reduced-memory mode kicks inside the force kernel).
-    const bool fuse = nsteps > 1 && !c->thermostat && c->rmm_dt == 0.0;
+    const bool fuse = nsteps > 1 && !c->thermostat && c->rmm_dt == 0.0 && !open_domain(c);
     auto kind_of = [&](int s) {
         if (!fuse) return (int)kStepFull;
         return s < nsteps - 1 ? (int)kStepFused : (int)kStepLast;
@@ -1610,16 +1735,22 @@ int mdx_cell_ids(mdx_context *c, int64_t n, int64_t *out) {
     if (!c || !out) return MDX_ERR_ARG;
     TRY(need(c, true, true, true));
     if (c->nrep > 1) return fail(c, MDX_ERR_ARG, "not available for batched replicas");
-    if (n != c->n) return fail(c, MDX_ERR_ARG, "n differs from the uploaded count");
     HIPCHK(c, hipSetDevice(c->device));
     TRY(ensure_binned(c));
-    std::vector<int> tmp((size_t)n), orig((size_t)n);
-    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->mls1id, (size_t)n * 4, hipMemcpyDeviceToHost,
+    const size_t N = c->n;
+    std::vector<int> tmp(N), orig(N), ls1(N);
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->mls1id, N * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(orig.data(), c->mol[c->cur].orig, N * 4, hipMemcpyDeviceToHost,
                              c->stream));
-    HIPCHK(c, hipMemcpyAsync(orig.data(), c->mol[c->cur].orig, (size_t)n * 4,
-                             hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int64_t i = 0; i < n; ++i) out[orig[i]] = tmp[i];
+    for (size_t i = 0; i < N; ++i) ls1[orig[i]] = tmp[i];
+    // deleted molecules (outflow walls) carry -1 and are left out
+    int64_t live = 0;
+    for (size_t i = 0; i < N; ++i) live += ls1[i] >= 0;
+    if (n != live) return fail(c, MDX_ERR_ARG, "n differs from the live molecule count");
+    int64_t k = 0;
+    for (size_t i = 0; i < N; ++i)
+        if (ls1[i] >= 0) out[k++] = ls1[i];
     return MDX_OK;
 }
 
@@ -1732,6 +1863,8 @@ int adjoint_check(mdx_context *c, int64_t n, const double *cot, const double *vj
     if (c->nrep > 1) return fail(c, MDX_ERR_ARG, "not available for batched replicas");
     if (c->nsites[T_D] > 0 || c->nsites[T_Q] > 0)
         return fail(c, MDX_ERR_ARG, "the adjoint covers LJ and charge sites only");
+    if (open_domain(c))
+        return fail(c, MDX_ERR_ARG, "the adjoint serves domains without outflow walls");
     if (c->model.nlj > kAdjMaxLJ)
         return fail(c, MDX_ERR_CAPACITY, "the adjoint supports <= 16 LJ centres");
     return MDX_OK;
@@ -1763,6 +1896,7 @@ int adjoint_run(mdx_context *c, const double *cot, double *dU_deps, double *dU_d
                            m.orig, c->dl_tmp, b.cot);
     }
     const int grid = adjoint_grid(c);
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     double *out_lj = b.out, *out_q = b.out + (size_t)kAdjMaxLJ * kAdjMaxLJ * 4;
     double *g_site[2] = {b.site, b.site + 3 * (size_t)nsl};
     const int ts[2] = {T_LJ, T_C};
@@ -1800,6 +1934,7 @@ int adjoint_run(mdx_context *c, const double *cot, double *dU_deps, double *dU_d
                            width, grid, t == T_LJ ? out_lj : out_q);
     }
     HIPCHK(c, hipGetLastError());
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     // host tables on the stack (no allocation): at most kAdjMaxLJ LJ ids
     double hl[kAdjMaxLJ * kAdjMaxLJ * 4], hq[kMaxComp * kMaxSite * 2];
     const size_t nhl = (size_t)nl * nl * 4, nhq = (size_t)ncomp * kMaxSite * 2;
@@ -1816,10 +1951,21 @@ int adjoint_run(mdx_context *c, const double *cot, double *dU_deps, double *dU_d
                            c->dl_tmp);
         hipLaunchKernelGGL(k_scatter_orig, dim3(nblk(nn)), dim3(kBlock), 0, c->stream, nn, m.orig,
                            c->dl_tmp, b.cot, nn, 3);
+    }
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    if (vjp)
         HIPCHK(c, hipMemcpyAsync(vjp, b.cot, (size_t)nn * 3 * 8, hipMemcpyDeviceToHost,
                                  c->stream));
-    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->timing) {
+        // kernel time: the derivative passes with their reductions, and up to
+        // the assembled VJP (transfers excluded)
+        float a0 = 0.f, a1 = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&a0, c->ev[0], c->ev[1]));
+        HIPCHK(c, hipEventElapsedTime(&a1, c->ev[0], c->ev[3]));
+        c->pair_ms = a0;
+        c->total_ms = a1;
+    }
     // each visit holds half of its pair's term under (a, b) = (own id, other
     // id): the symmetric parameter's derivative sums both orders
     double *outs[4] = {dU_deps, dU_dsig, dW_deps, dW_dsig};
@@ -1866,11 +2012,28 @@ int mdx_set_counting(mdx_context *c, int enable) {
 
 int mdx_counters(mdx_context *c, int64_t *out) {
     if (!c || !out) return MDX_ERR_ARG;
-    unsigned long long h[8];
+    unsigned long long h[8 * kTypes];
     HIPCHK(c, hipMemcpyAsync(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     // owner computes: every test and every pair was made from both sides
-    for (int k = 0; k < 8; ++k) out[k] = (int64_t)(h[k] / 2);
+    for (int k = 0; k < 8; ++k) {
+        unsigned long long sum = 0;
+        for (int t = 0; t < kTypes; ++t) sum += h[8 * t + k];
+        out[k] = (int64_t)(sum / 2);
+    }
+    return MDX_OK;
+}
+
+int mdx_kernel_stats(mdx_context *c, double *ms, int64_t *counters) {
+    if (!c) return MDX_ERR_ARG;
+    if (ms)
+        for (int t = 0; t < kTypes; ++t) ms[t] = c->kernel_ms[t];
+    if (counters) {
+        unsigned long long h[8 * kTypes];
+        HIPCHK(c, hipMemcpyAsync(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int k = 0; k < 8 * kTypes; ++k) counters[k] = (int64_t)h[k];
+    }
     return MDX_OK;
 }
 

@@ -50,6 +50,9 @@ struct Geom {
     int cpd[3];        // ls1 cells per dimension (incl. halo)
     int pdim[3];       // periodic per dimension (images across the box)
     int reflect[3];    // reflecting walls per dimension (BoundaryType::REFLECTING)
+    int outflow[3];    // outflow walls per dimension (BoundaryType::OUTFLOW): leaving
+                       // molecules are deleted (their position becomes NaN and the
+                       // next re-bin moves them behind the last cell)
     double wall_rc;    // width of the wall slab checked for leaving molecules (rc)
     double rc2, rclj2, epsRFInvrc3;
 };

@@ -156,27 +156,43 @@ __device__ __forceinline__ void lf_pre_mol(MolArrays &s, const double *__restric
     const double dt_halve = .5 * dt;
     const double dtInv2m = dt_halve / c.m;
     double r[3] = {s.rx[i], s.ry[i], s.rz[i]};
-    if (f && (g.reflect[0] | g.reflect[1] | g.reflect[2])) {
-        // BoundaryHandler::processGlobalWallLeavingParticles, reflecting walls
-        // (BoundaryHandler.cpp:82-134; called before upd_preF,
-        // Simulation.cpp:1114), walls in the order +x, -x, +y, -y, +z, -z: a
-        // molecule in the wall's slab of width rc (RegionUtils.cpp:11-45,
-        // inBox l <= r < u) whose predicted position r + dt (v + dt/2m F)
-        // reaches the wall (RegionUtils.cpp:47-56) gets v = -v - 2 dt/2m F,
-        // so the coming half kick leaves it moving away at -(v + dt/2m F)
+    if (r[0] != r[0]) return;  // deleted at an outflow wall: no longer moved
+    if (f && (g.reflect[0] | g.reflect[1] | g.reflect[2] | g.outflow[0] | g.outflow[1] |
+              g.outflow[2])) {
+        // BoundaryHandler::processGlobalWallLeavingParticles (BoundaryHandler.cpp:
+        // 82-134; called before upd_preF, Simulation.cpp:1114), walls in the
+        // order +x, -x, +y, -y, +z, -z: a molecule in the wall's slab of width
+        // rc (RegionUtils.cpp:11-45, inBox l <= r < u) whose predicted position
+        // r + dt (v + dt/2m F) reaches the wall (RegionUtils.cpp:47-56) gets
+        // v = -v - 2 dt/2m F at a reflecting wall, so the coming half kick
+        // leaves it moving away at -(v + dt/2m F), and is deleted at an
+        // outflow wall (later walls no longer see it)
         bool inside = true;
         for (int d = 0; d < 3; ++d) inside = inside && r[d] >= g.bbmin[d] && r[d] < g.bbmax[d];
-        for (int d = 0; d < 3 && inside; ++d) {
-            if (!g.reflect[d]) continue;
+        bool dead = false;
+        for (int d = 0; d < 3 && inside && !dead; ++d) {
+            if (!g.reflect[d] && !g.outflow[d]) continue;
             const double adj = dtInv2m * f[d];
             // +wall: slab [max(max - rc, min), max)
             if (r[d] >= fmax(g.bbmax[d] - g.wall_rc, g.bbmin[d]) &&
-                r[d] + dt * (v[d] + adj) >= g.bbmax[d])
-                v[d] = -v[d] - adj - adj;
+                r[d] + dt * (v[d] + adj) >= g.bbmax[d]) {
+                if (g.reflect[d])
+                    v[d] = -v[d] - adj - adj;
+                else
+                    dead = true;
+            }
             // -wall: slab [min, min(min + rc, max))
-            if (r[d] < fmin(g.bbmin[d] + g.wall_rc, g.bbmax[d]) &&
-                r[d] + dt * (v[d] + adj) <= g.bbmin[d])
-                v[d] = -v[d] - adj - adj;
+            if (!dead && r[d] < fmin(g.bbmin[d] + g.wall_rc, g.bbmax[d]) &&
+                r[d] + dt * (v[d] + adj) <= g.bbmin[d]) {
+                if (g.reflect[d])
+                    v[d] = -v[d] - adj - adj;
+                else
+                    dead = true;
+            }
+        }
+        if (dead) {
+            s.rx[i] = s.ry[i] = s.rz[i] = __builtin_nan("");
+            return;
         }
     }
     for (int d = 0; d < 3; ++d) {
@@ -270,10 +286,13 @@ __global__ __launch_bounds__(256) void k_leapfrog_post(MolArrays s, const double
                                                        double dt, int n,
                                                        const int *__restrict__ chunk,
                                                        const int *__restrict__ chunk_end,
-                                                       double *partial, int rmm) {
+                                                       double *partial, int rmm, int open) {
     int i = chunk[blockIdx.x] + threadIdx.x;
     double mv2 = 0, iw2 = 0, cnt = 0, rot = 0;
-    if (rmm && i < chunk_end[blockIdx.x]) {
+    const bool deleted = open && i < chunk_end[blockIdx.x] && s.rx[i] != s.rx[i];
+    if (deleted) {
+        // deleted at an outflow wall: not moved, not counted
+    } else if (rmm && i < chunk_end[blockIdx.x]) {
         // VelocityCellProcessorRMM (VelocityCellProcessorRMM.h:60-110): the
         // kick happened in the force kernel; sum v^2 (no mass factor, as the
         // reference's reduced-memory mode sums it) and N, no rotation
@@ -367,14 +386,21 @@ __global__ __launch_bounds__(256) void k_bin_count(MolArrays s, Reps R, int n, i
     const bool in = i < n;
     int c = -1;
     if (in) {
-        const int rep = mol_replica(R, i);
-        const Geom &g = R.geom[rep];
         double p[3] = {s.rx[i], s.ry[i], s.rz[i]};
-        int ci[3];
-        ls1_cell3(g, p, ci);
-        c = R.cell_off[rep] + inner_index(g, ci);
+        if (p[0] != p[0] && R.nrep == 1) {
+            // deleted at an outflow wall: the bin behind the last cell, which
+            // no traversal visits
+            c = R.cell_off[1];
+            ls1id[i] = -1;
+        } else {
+            const int rep = mol_replica(R, i);
+            const Geom &g = R.geom[rep];
+            int ci[3];
+            ls1_cell3(g, p, ci);
+            c = R.cell_off[rep] + inner_index(g, ci);
+            ls1id[i] = (ci[2] * g.cpd[1] + ci[1]) * g.cpd[0] + ci[0];
+        }
         cell[i] = c;
-        ls1id[i] = (ci[2] * g.cpd[1] + ci[1]) * g.cpd[0] + ci[0];
     }
     const int cprev = __shfl_up(c, 1, 64);
     const bool head = in && (lane == 0 || cprev != c);
@@ -489,7 +515,7 @@ __global__ __launch_bounds__(256) void k_bin_gather_point(
     const int d = stable_dest(p, n, cell, perm, start, win, &i, &c);
     if (d < 0) return;
     gather_common(i, d, c, src, dst, newcell, ls1id, newls1id);
-    const int rep = cell_replica(R, c);
+    const int rep = c < R.cell_off[R.nrep] ? cell_replica(R, c) : 0;  // deleted: the last bin
     const Geom &g = R.geom[rep];
     const int local = c - R.cell_off[rep];
     const int cx = local % g.n[0], cy = (local / g.n[0]) % g.n[1], cz = local / (g.n[0] * g.n[1]);

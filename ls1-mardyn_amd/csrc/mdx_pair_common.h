@@ -234,6 +234,17 @@ __device__ __forceinline__ Nbr neighbour(const Geom &g, int cx, int cy, int cz, 
     return b;
 }
 
+// Neighbour index of tile entry idx (concatenated neighbour order): the
+// largest k with pre[k] <= idx.
+__device__ __forceinline__ int tile_k(const int *pre, int idx) {
+    int lo = 0, hi = 27;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (pre[mid] <= idx) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 template <typename R>
 __device__ __forceinline__ V3<R> image_offset(const Geom &g, const Nbr &b);
 

@@ -17,26 +17,47 @@
 //            corner (|x| <= ~cell length, so FP32 keeps ~1e-7 absolute
 //            accuracy at any box size); the cutoff decision is re-made in FP64
 //            when the FP32 r^2 lies within 1e-5 relative of rc^2 so the pair
-//            set equals the FP64 one; every per-pair result is converted to
-//            FP64 before it is summed (RealAccumVec in the reference's MIXED
-//            build, RealAccumVecSPDP.h).
+//            set equals the FP64 one; the pair terms of one drained hit list
+//            (at most a list length of partners per lane) are summed in FP32
+//            and added to the site's FP64 accumulators once per drain, so
+//            every per-site and per-molecule sum across drains, cells and
+//            partner types is FP64 (the reference's MIXED build accumulates
+//            in RealAccumVec, RealAccumVecSPDP.h).
 #include "mdx_pair_common.h"
 
 namespace mdx {
 
 template <typename R> __device__ __forceinline__ V3<R> v3(const Q4<R> &q) { return {q.x, q.y, q.z}; }
 
+// FP32 form: partial sums of one drained hit list, folded into the FP64
+// accumulators by flush().
+struct Part32 {
+    float f[3], v[3], m[3], u, rf;
+    __device__ __forceinline__ void clear() {
+        for (int k = 0; k < 3; ++k) f[k] = v[k] = m[k] = 0.f;
+        u = rf = 0.f;
+    }
+    __device__ __forceinline__ void flush(Acc &a) const {
+        for (int k = 0; k < 3; ++k) {
+            a.f[k] += (double)f[k];
+            a.v[k] += (double)v[k];
+            a.m[k] += (double)m[k];
+        }
+        a.u += (double)u;
+        a.rf += (double)rf;
+    }
+};
+
 // LJ pair seen from site i, evaluated on every lane of the wave: a lane
 // whose entry is not accepted (ok false) runs the body at a huge r^2 (FP32:
 // +inf; FP64: 1e300, which keeps the Newton steps of 1/r^2 finite), where
 // force and virial come out exactly 0, and adds no energy; u = 6 U_LJ +
-// shift6 (_loopBodyLJ, VectorizedCellProcessor.cpp:171-224). FP32 form: pair terms
-// in FP32 up to the radial factor, the force and virial products formed in
-// FP64 from exact conversions.
+// shift6 (_loopBodyLJ, VectorizedCellProcessor.cpp:171-224). FP64 form: into
+// acc; FP32 form: into the drain's FP32 partial sums pa.
 template <typename R, bool MOLSITE, bool MULTI>
 __device__ __forceinline__ void eval_lj(const PairArgs &a, const ISite<R> &I, const Q4<R> &pj,
                                         const Q4<R> &mj, bool ok, R eps24, R sig2, R shift6,
-                                        Acc &acc) {
+                                        Acc &acc, Part32 &pa) {
     const V3<R> d = {I.pos.x - pj.x, I.pos.y - pj.y, I.pos.z - pj.z};
     V3<R> dm = d;
     if constexpr (!MOLSITE) dm = {I.mol.x - mj.x, I.mol.y - mj.y, I.mol.z - mj.z};
@@ -52,15 +73,26 @@ __device__ __forceinline__ void eval_lj(const PairArgs &a, const ISite<R> &I, co
     const R r2inv = Math<R>::rcp(ok ? dot3(d, d) : far);
     const R lj2 = sig2 * r2inv, lj6 = lj2 * lj2 * lj2, lj12 = lj6 * lj6;
     const R lj12m6 = lj12 - lj6;
-    const double s = (double)(eps24 * r2inv * (lj12 + lj12m6));
-    const double fx = (double)d.x * s, fy = (double)d.y * s, fz = (double)d.z * s;
-    acc.f[0] += fx;
-    acc.f[1] += fy;
-    acc.f[2] += fz;
-    acc.v[0] += (double)dm.x * fx;
-    acc.v[1] += (double)dm.y * fy;
-    acc.v[2] += (double)dm.z * fz;
-    acc.u += (double)(ok ? eps24 * lj12m6 + shift6 : R(0));
+    const R s = eps24 * r2inv * (lj12 + lj12m6);
+    const R fx = d.x * s, fy = d.y * s, fz = d.z * s;
+    const R u = ok ? eps24 * lj12m6 + shift6 : R(0);
+    if constexpr (sizeof(R) == 4) {
+        pa.f[0] += fx;
+        pa.f[1] += fy;
+        pa.f[2] += fz;
+        pa.v[0] += dm.x * fx;
+        pa.v[1] += dm.y * fy;
+        pa.v[2] += dm.z * fz;
+        pa.u += u;
+    } else {
+        acc.f[0] += fx;
+        acc.f[1] += fy;
+        acc.f[2] += fz;
+        acc.v[0] += dm.x * fx;
+        acc.v[1] += dm.y * fy;
+        acc.v[2] += dm.z * fz;
+        acc.u += u;
+    }
 }
 
 // One electrostatic site pair seen from site i (tile entry s): force on i, virial
@@ -71,7 +103,7 @@ template <int TI, int TJ, typename R, bool MOLSITE, bool MULTI>
 __device__ __forceinline__ void eval_pair(const PairArgs &a, const ISite<R> &I,
                                           const Tile<TI, R, MOLSITE, MULTI> &T, int s,
                                           const Q4<R> &pj, const Q4<R> &mj, R eps24, R sig2,
-                                          R shift6, Acc &acc) {
+                                          R shift6, Acc &acc, Part32 &pa) {
     V3<R> d = {I.pos.x - pj.x, I.pos.y - pj.y, I.pos.z - pj.z};
     V3<R> dm = d;
     if constexpr (!MOLSITE) dm = {I.mol.x - mj.x, I.mol.y - mj.y, I.mol.z - mj.z};
@@ -115,31 +147,35 @@ __device__ __forceinline__ void eval_pair(const PairArgs &a, const ISite<R> &I,
         tq = o.M1;
     }
     V3<R> f = {sign * o.f.x, sign * o.f.y, sign * o.f.z};
-    acc.f[0] += (double)f.x;
-    acc.f[1] += (double)f.y;
-    acc.f[2] += (double)f.z;
-    double vx = (double)(dm.x * f.x), vy = (double)(dm.y * f.y), vz = (double)(dm.z * f.z);
-    acc.v[0] += vx;
-    acc.v[1] += vy;
-    acc.v[2] += vz;
-    acc.u += (double)o.u;
-    if constexpr (TI == T_D && TJ == T_D) acc.rf += (double)o.rf;
-    if constexpr (TI >= T_D) {
-        acc.m[0] += (double)tq.x;
-        acc.m[1] += (double)tq.y;
-        acc.m[2] += (double)tq.z;
+    if constexpr (sizeof(R) == 4) {
+        pa.f[0] += f.x;
+        pa.f[1] += f.y;
+        pa.f[2] += f.z;
+        pa.v[0] += dm.x * f.x;
+        pa.v[1] += dm.y * f.y;
+        pa.v[2] += dm.z * f.z;
+        pa.u += o.u;
+        if constexpr (TI == T_D && TJ == T_D) pa.rf += o.rf;
+        if constexpr (TI >= T_D) {
+            pa.m[0] += tq.x;
+            pa.m[1] += tq.y;
+            pa.m[2] += tq.z;
+        }
+    } else {
+        acc.f[0] += f.x;
+        acc.f[1] += f.y;
+        acc.f[2] += f.z;
+        acc.v[0] += dm.x * f.x;
+        acc.v[1] += dm.y * f.y;
+        acc.v[2] += dm.z * f.z;
+        acc.u += o.u;
+        if constexpr (TI == T_D && TJ == T_D) acc.rf += o.rf;
+        if constexpr (TI >= T_D) {
+            acc.m[0] += tq.x;
+            acc.m[1] += tq.y;
+            acc.m[2] += tq.z;
+        }
     }
-}
-
-// Neighbour index of tile entry idx (concatenated neighbour order): the
-// largest k with pre[k] <= idx.
-__device__ __forceinline__ int tile_k(const int *pre, int idx) {
-    int lo = 0, hi = 27;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (pre[mid] <= idx) lo = mid; else hi = mid;
-    }
-    return lo;
 }
 
 constexpr int kUnroll = 4;  // stage-A candidates in flight per lane
@@ -160,6 +196,12 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
     const SiteArrays &J = a.s[TJ];
     const double rc2 = (TI == T_LJ) ? a.rclj2 : a.rc2;
     const R rc2r = (R)rc2;
+    // FP32 LJ tiles: the test record's w holds |x|^2 and stage A computes
+    // r^2 = (|i|^2 + |j|^2) - 2 i.j (4 instead of 6 operations); its rounding
+    // stays below 1.5e-5 (cl / rc)^2 rc^2 (coordinates within two cell edges
+    // of the i cell's corner), so stage A keeps candidates up to that much
+    // above rc^2 and stage B re-decides the band in FP64 as before
+    constexpr bool NORM = TI == T_LJ && sizeof(R) == 4 && (C::MOL || !C::LJID);
     // idle lanes test from far away: r^2 overflows or exceeds any cutoff
     const V3<R> far = {R(1e30), R(1e30), R(1e30)};
     const V3<R> Imol = valid ? I.mol : far, Ipos = valid ? I.pos : far;
@@ -231,6 +273,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             q.w = R(0);
             if constexpr (C::LJID) q.w = (R)J.ljid[j];
             if constexpr (TI != T_LJ) q.w = SiteView<R>::par(J, j);
+            if constexpr (NORM && !C::MOL) q.w = q.x * q.x + q.y * q.y + q.z * q.z;
             T.p[s] = q;
             if constexpr (C::MOL) {
                 const V3<R> m = SiteView<R>::mol(J, j);
@@ -243,7 +286,9 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                     const int mo = J.mol[j];
                     w = (R)(J.mstart[mo + 1] - J.mstart[mo]);
                 }
-                T.m[s] = Q4<R>{m.x + off.x, m.y + off.y, m.z + off.z, w};
+                Q4<R> mq{m.x + off.x, m.y + off.y, m.z + off.z, w};
+                if constexpr (NORM) mq.w = mq.x * mq.x + mq.y * mq.y + mq.z * mq.z;
+                T.m[s] = mq;
             }
             if constexpr (TI != T_LJ && TJ >= T_D) {
                 const V3<R> e = SiteView<R>::ori(J, j);
@@ -285,25 +330,49 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             return true;
         };
         auto drain = [&]() {
+            Part32 pa;
+            pa.clear();
             if constexpr (TI == T_LJ) {
                 // LJ: every lane runs every entry up to the longest list
                 // (exhausted lanes on entry 0 with the pair masked off), so
-                // the loop has no divergent branch but the FP32 form's rare
-                // re-check
-                for (int e = 0; __any(e < cnt); ++e) {
+                // the loop has no branch. FP32: entries in the re-check band
+                // around rc^2 are masked off here and marked in band (bit e),
+                // and re-decided in FP64 after the loop (rare)
+                unsigned band = 0u;
+                // the longest list, as a wave-uniform (scalar) trip count
+                int emax = cnt;
+                for (int o = 32; o > 0; o >>= 1) emax = max(emax, __shfl_xor(emax, o, 64));
+                emax = __builtin_amdgcn_readfirstlane(emax);
+                for (int e = 0; e < emax; ++e) {
                     const bool act = e < cnt;
-                    const int sj = act ? (int)T.lst[e][lane] : 0;
+                    const int raw = (int)T.lst[e][lane];  // e < LIST: in bounds
+                    const int sj = act ? raw : 0;
                     const Q4<R> qj = C::MOL ? T.m[sj] : T.p[sj];
                     const V3<R> mi = C::MOL ? I.mol : I.pos;
                     const R dx = mi.x - qj.x, dy = mi.y - qj.y, dz = mi.z - qj.z;
                     const R r2 = dx * dx + dy * dy + dz * dz;
                     bool ok = act && r2 != R(0);
                     if constexpr (sizeof(R) == 4) {
-                        if (ok && r2 >= lo) ok = recheck(sj);
+                        const bool b = ok && r2 >= lo;
+                        band |= b ? (1u << e) : 0u;
+                        ok = ok && !b;
                     }
                     const Q4<R> pj = C::MOL ? T.p[sj] : qj;
-                    eval_lj<R, MOLSITE, MULTI>(a, I, pj, qj, ok, eps24, sig2, shift6, acc);
+                    eval_lj<R, MOLSITE, MULTI>(a, I, pj, qj, ok, eps24, sig2, shift6, acc, pa);
                     if (COUNT) nhit += ok;
+                }
+                if constexpr (sizeof(R) == 4) {
+                    while (__any(band != 0u)) {
+                        const bool has = band != 0u;
+                        const int e = has ? __builtin_ctz(band) : 0;
+                        band &= has ? band - 1u : 0u;
+                        const int sj = has ? (int)T.lst[e][lane] : 0;
+                        const bool ok = has && recheck(sj);
+                        const Q4<R> qj = C::MOL ? T.m[sj] : T.p[sj];
+                        const Q4<R> pj = C::MOL ? T.p[sj] : qj;
+                        eval_lj<R, MOLSITE, MULTI>(a, I, pj, qj, ok, eps24, sig2, shift6, acc, pa);
+                        if (COUNT) nhit += ok;
+                    }
                 }
             } else {
                 // the next entry's index and cutoff record are read one
@@ -329,18 +398,19 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                             for (int t = 0; t < ns; ++t) {
                                 const Q4<R> pj = T.p[sj + t];
                                 eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj + t, pj, qj,
-                                                                     eps24, sig2, shift6, acc);
+                                                                     eps24, sig2, shift6, acc, pa);
                             }
                             if (COUNT) nhit += ns;
                         } else {
                             const Q4<R> pj = C::MOL ? T.p[sj] : qj;
                             eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj, pj, qj, eps24,
-                                                                 sig2, shift6, acc);
+                                                                 sig2, shift6, acc, pa);
                             if (COUNT) ++nhit;
                         }
                     }
                 }
             }
+            if constexpr (sizeof(R) == 4) pa.flush(acc);
             cnt = 0;
         };
         // Stage A: keep candidates below the cutoff (FP32: up to the top of
@@ -348,7 +418,8 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         // full are inside [0, nst) for every phase and need no bounds test,
         // and idle lanes carry a far-away position, so the bulk loop is a
         // read, a distance and a compare per candidate.
-        const int per = (nh + P - 1) / P, full = per - 1;
+        // wave-uniform trip counts (scalar loop control)
+        const int per = __builtin_amdgcn_readfirstlane((nh + P - 1) / P), full = per - 1;
         const Q4<R> *rec = (C::MOL ? T.m : T.p) + ph;
         // candidate c of the stage-A sequence -> tile index
         auto cand = [&](int c) -> int {
@@ -356,7 +427,26 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             else return c;
         };
         const V3<R> mi = C::MOL ? Imol : Ipos;
-        const R cut = sizeof(R) == 4 ? hi : rc2r;
+        R cut = sizeof(R) == 4 ? hi : rc2r;
+        // NORM: -2 i and |i|^2 per lane (idle lanes: far away, never a hit)
+        R mi2 = R(0), nx = R(0), ny = R(0), nz = R(0);
+        if constexpr (NORM) {
+            const double clm = fmax(g.cl[0], fmax(g.cl[1], g.cl[2]));
+            const double band = fmax(1e-5, 1.5e-5 * clm * clm / rc2);
+            cut = (R)(rc2 * (1.0 + band));
+            mi2 = mi.x * mi.x + mi.y * mi.y + mi.z * mi.z;
+            nx = R(-2) * mi.x;
+            ny = R(-2) * mi.y;
+            nz = R(-2) * mi.z;
+        }
+        auto dist2 = [&](const Q4<R> &q) -> R {
+            if constexpr (NORM) {
+                return __builtin_fmaf(nx, q.x, __builtin_fmaf(ny, q.y, __builtin_fmaf(nz, q.z, mi2 + q.w)));
+            } else {
+                const R dx = mi.x - q.x, dy = mi.y - q.y, dz = mi.z - q.z;
+                return dx * dx + dy * dy + dz * dz;
+            }
+        };
         auto test = [&](R r2) { return sizeof(R) == 4 ? r2 <= cut : r2 < cut; };
         int m0 = 0;
         for (; m0 + kUnroll <= full; m0 += kUnroll) {
@@ -374,8 +464,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                     sv[u] = ph + (m0 + u) * P;
                     q = rec[(m0 + u) * P];
                 }
-                const R dx = mi.x - q.x, dy = mi.y - q.y, dz = mi.z - q.z;
-                r2v[u] = dx * dx + dy * dy + dz * dz;
+                r2v[u] = dist2(q);
             }
             if (COUNT && valid) ntest += kUnroll;
             // every candidate is written at the end of the list and kept only
@@ -392,8 +481,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             const int c = ph + m0 * P;
             const int sidx = cand(min(c, nh - 1));
             const Q4<R> q = C::MOL ? T.m[sidx] : T.p[sidx];
-            const R dx = mi.x - q.x, dy = mi.y - q.y, dz = mi.z - q.z;
-            const R r2 = dx * dx + dy * dy + dz * dz;
+            const R r2 = dist2(q);
             const bool in = valid && c < nh;
             if (COUNT && in) ++ntest;
             T.lst[cnt][lane] = (unsigned short)sidx;

@@ -32,6 +32,7 @@ EXPORTS = (
     "mdx_upload_device", "mdx_set_timing", "mdx_last_kernel_ms", "mdx_set_charges",
     "mdx_adjoint", "mdx_lj_tables", "mdx_set_replicas", "mdx_set_pair_config", "mdx_autotune",
     "mdx_set_thermostat", "mdx_thermostat_state", "mdx_set_rmm", "mdx_set_boundaries",
+    "mdx_molecule_count", "mdx_kernel_stats",
 )
 
 _lib = None
@@ -84,6 +85,8 @@ def load():
     L.mdx_set_rmm.argtypes = [vp, C.c_double]
     L.mdx_set_boundaries.argtypes = [vp, _ip]
     L.mdx_adjoint.argtypes = [vp, C.c_int64] + [vp] * 8
+    L.mdx_molecule_count.argtypes = [vp, vp]
+    L.mdx_kernel_stats.argtypes = [vp, vp, vp]
     _lib = L
     return L
 
@@ -105,6 +108,7 @@ class Context:
         self.h = h
         self.n = 0
         self.nrep = 1
+        self.open = False
 
     def _chk(self, rc, what):
         if rc != 0:
@@ -142,6 +146,7 @@ class Context:
                                         float(rc_lj), int(bool(periodic)), int(cell_mode)),
                   "mdx_set_domain")
         self.nrep = 1
+        self.open = False  # a new domain has periodic boundaries
         self.set_precision(precision)
         return self
 
@@ -184,9 +189,11 @@ class Context:
                   "mdx_set_thermostat")
 
     def set_boundaries(self, types):
-        """Per-dimension boundary types: BOUNDARY_PERIODIC or BOUNDARY_REFLECTING."""
+        """Per-dimension boundary types: BOUNDARY_PERIODIC, BOUNDARY_OUTFLOW or
+        BOUNDARY_REFLECTING."""
         t = np.ascontiguousarray(types, np.int32)
         self._chk(self.L.mdx_set_boundaries(self.h, t), "mdx_set_boundaries")
+        self.open = bool(np.any(t == BOUNDARY_OUTFLOW))
 
     def set_rmm(self, dt):
         """Reduced-memory one-centre mode (half-step velocities, kick in the
@@ -234,8 +241,16 @@ class Context:
         self._chk(self.L.mdx_upload(self.h, n, f(r, 3), f(v, 3), f(q, 4), f(L, 3),
                                     np.ascontiguousarray(cid, np.int32), idp), "mdx_upload")
 
+    def molecule_count(self):
+        """Live molecules: the uploaded count less those deleted at outflow walls."""
+        k = C.c_int64(0)
+        self._chk(self.L.mdx_molecule_count(self.h, C.byref(k)), "mdx_molecule_count")
+        return k.value
+
     def download(self):
-        n = self.n
+        """State in upload order (with outflow walls: the surviving molecules,
+        ids tell which)."""
+        n = self.molecule_count() if self.open else self.n
         out = dict(r=np.zeros((n, 3)), v=np.zeros((n, 3)), q=np.zeros((n, 4)), L=np.zeros((n, 3)),
                    F=np.zeros((n, 3)), M=np.zeros((n, 3)), Vi=np.zeros((n, 3)),
                    ids=np.zeros(n, np.uint64))
@@ -282,8 +297,8 @@ class Context:
         self._chk(self.L.mdx_synchronize(self.h), "mdx_synchronize")
 
     def cell_ids(self):
-        out = np.zeros(self.n, np.int64)
-        self._chk(self.L.mdx_cell_ids(self.h, self.n, out), "mdx_cell_ids")
+        out = np.zeros(self.molecule_count() if self.open else self.n, np.int64)
+        self._chk(self.L.mdx_cell_ids(self.h, len(out), out), "mdx_cell_ids")
         return out
 
     def cell_geometry(self):
@@ -310,6 +325,15 @@ class Context:
 
     def set_timing(self, on):
         self._chk(self.L.mdx_set_timing(self.h, int(bool(on))), "mdx_set_timing")
+
+    def kernel_stats(self):
+        """Per pair kernel (LJ, charge, dipole, quadrupole i sites): the last
+        timed traversal's durations [4] (ms) and the last counted traversal's
+        raw visit counts [4, 8] (each kernel counts its own side of a pair)."""
+        ms = np.zeros(4)
+        cnt = np.zeros((4, 8), np.int64)
+        self._chk(self.L.mdx_kernel_stats(self.h, _ptr(ms), _ptr(cnt)), "mdx_kernel_stats")
+        return ms, cnt
 
     def last_kernel_ms(self):
         a, b = C.c_double(0), C.c_double(0)

@@ -30,6 +30,7 @@
 #include "ls1_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -56,12 +57,14 @@ typedef struct {
     int ncomp;
     ox_comp comp[OX_MAXCOMP];
     int nlj_total;
+    int nthreads;   /* OpenMP threads of the traversal and the molecule loops (1: serial) */
     double *eps24, *sig2, *shift6; /* nlj_total x nlj_total */
     double epsRF, rc, rcLJ, epsRFInvrc3;
     double bbmin[3], bbmax[3];
     int periodic;   /* any dimension periodic */
     int pdim[3];    /* periodic per dimension */
     int wall[3];    /* reflecting walls per dimension */
+    int outflow[3]; /* outflow walls per dimension */
     /* LinkedCells geometry */
     int boxcells[3], cpd[3];
     double cell_len[3], cell_rlen[3], halo_min[3], halo_max[3];
@@ -249,6 +252,7 @@ void *ox_create(int ncomp, const double *desc, int desc_len, const double *mix,
     x->epsRF = epsRF;
     x->rc = rc;
     x->rcLJ = rcLJ;
+    x->nthreads = 1;
     /* VectorizedCellProcessor ctor, VectorizedCellProcessor.cpp:19-23 */
     x->epsRFInvrc3 = 2. * (epsRF - 1.) / ((rc * rc * rc) * (2. * epsRF + 1.));
     for (int d = 0; d < 3; ++d) {
@@ -259,6 +263,7 @@ void *ox_create(int ncomp, const double *desc, int desc_len, const double *mix,
     for (int d = 0; d < 3; ++d) {
         x->pdim[d] = periodic ? 1 : 0;
         x->wall[d] = 0;
+        x->outflow[d] = 0;
     }
     build_lj_tables(x, mix);
     if (build_geometry(x, cell_mode) < 0) {
@@ -923,12 +928,63 @@ static int make_halo(const ox_ctx *x, ox_mol **pm, int n, int *cap) {
     return total;
 }
 
+/* The C08 walk with the colours in order and the base cells of one colour
+ * shared among OpenMP threads, as ls1's C08CellPairTraversal does
+ * (C08CellPairTraversal.h:45-75): cells of one colour's base blocks are
+ * disjoint, so every molecule's sums keep their order; the macroscopic sums
+ * and counters are per thread, added in thread order at the end. */
+static void walk_c08_threads(const ox_ctx *x, trav_state *ts) {
+    const int nt = x->nthreads;
+    double *mac = (double *)calloc((size_t)nt * 4, sizeof(double));
+    long long *cnt = (long long *)calloc((size_t)nt * OX_NCNT, sizeof(long long));
+    for (int col = 0; col < 8; ++col) {
+        const int s0 = col & 1, s1 = (col >> 1) & 1, s2 = (col >> 2) & 1;
+        const int nx = (x->cpd[0] - 1 - s0 + 1) / 2, ny = (x->cpd[1] - 1 - s1 + 1) / 2,
+                  nz = (x->cpd[2] - 1 - s2 + 1) / 2;
+        const long nb = (long)nx * ny * nz;
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 16)
+        for (long b = 0; b < nb; ++b) {
+            const int tid = omp_get_thread_num();
+            trav_state t = *ts;
+            t.mac = mac + 4 * tid;
+            t.cnt = ts->cnt ? cnt + OX_NCNT * tid : NULL;
+            const int i = s0 + 2 * (int)(b % nx), j = s1 + 2 * (int)((b / nx) % ny),
+                      k = s2 + 2 * (int)(b / ((long)nx * ny));
+            for (int p = 0; p < 14; ++p) {
+                const int *a = c08_pairs[p][0], *bb = c08_pairs[p][1];
+                long c1 = lin3(x, i + a[0], j + a[1], k + a[2]);
+                long c2 = lin3(x, i + bb[0], j + bb[1], k + bb[2]);
+                int h1 = is_halo_cell(x, c1), h2 = is_halo_cell(x, c2);
+                if (h1 && h2) continue;
+                if (c1 != c2 && h1) {
+                    long tt = c1;
+                    c1 = c2;
+                    c2 = tt;
+                }
+                trav_cb(&t, c1, c2);
+            }
+        }
+    }
+    for (int t = 0; t < nt; ++t) {
+        for (int k = 0; k < 4; ++k) ts->mac[k] += mac[4 * t + k];
+        if (ts->cnt)
+            for (int k = 0; k < OX_NCNT; ++k) ts->cnt[k] += cnt[OX_NCNT * t + k];
+    }
+    free(mac);
+    free(cnt);
+}
+
+void ox_set_threads(void *h, int nthreads) {
+    ((ox_ctx *)h)->nthreads = nthreads > 1 ? nthreads : 1;
+}
+
 int ox_forces(void *h, int n, const double *r, double *q, const int *cid, double *F,
               double *M, double *Vi, double *macro, long long *counters) {
     ox_ctx *x = (ox_ctx *)h;
     int cap = n + 16 + n / 2;
     ox_mol *mols = (ox_mol *)malloc(sizeof(ox_mol) * (size_t)cap);
     if (!mols) return -1;
+#pragma omp parallel for num_threads(x->nthreads) schedule(static) if (x->nthreads > 1)
     for (int i = 0; i < n; ++i) {
         setup_mol(x, &mols[i], r + 3 * i, q + 4 * i, cid[i], i);
         for (int k = 0; k < 4; ++k) q[4 * i + k] = mols[i].q[k];
@@ -950,7 +1006,11 @@ int ox_forces(void *h, int n, const double *r, double *q, const int *cid, double
     long long cnt[OX_NCNT];
     memset(cnt, 0, sizeof(cnt));
     trav_state ts = {x, mols, cs, cl, mac, cnt};
-    walk_c08(x, trav_cb, &ts);
+    if (x->nthreads > 1)
+        walk_c08_threads(x, &ts);
+    else
+        walk_c08(x, trav_cb, &ts);
+#pragma omp parallel for num_threads(x->nthreads) schedule(static) if (x->nthreads > 1)
     for (int i = 0; i < n; ++i) calc_fm(x, &mols[i], F + 3 * i, M + 3 * i, Vi + 3 * i);
     for (int k = 0; k < 4; ++k) macro[k] = mac[k];
     if (counters)
@@ -958,6 +1018,59 @@ int ox_forces(void *h, int n, const double *r, double *q, const int *cid, double
     free(fill);
     free(ci);
     free(cl);
+    free(cs);
+    free(mols);
+    return 0;
+}
+
+/* Molecule-centre distance checks of the reference's traversal
+ * (FlopCounter::processCell / processCellPair, FlopCounter.cpp:206-305: n (n-1)
+ * / 2 per cell, n1 n2 per cell pair, over the C08 walk with the halo copies):
+ * out[0] = every check the reference makes, including the second visit of a
+ * boundary-halo pair from the other periodic side (processed without
+ * macroscopic sums), out[1] = the checks of the pairs processed with them
+ * (each unordered molecule pair once, the basis of the flop count here). No
+ * forces are evaluated. */
+typedef struct {
+    const ox_ctx *x;
+    const int *cell_start;
+    long long all, macro;
+} count_state;
+
+static void count_cb(void *u, long c1, long c2) {
+    count_state *t = (count_state *)u;
+    const long long n1 = t->cell_start[c1 + 1] - t->cell_start[c1];
+    if (c1 == c2) {
+        if (is_halo_cell(t->x, c1) || n1 < 2) return;
+        t->all += n1 * (n1 - 1) / 2;
+        t->macro += n1 * (n1 - 1) / 2;
+        return;
+    }
+    const long long n2 = t->cell_start[c2 + 1] - t->cell_start[c2];
+    if (n1 == 0 || n2 == 0) return;
+    const int h1 = is_halo_cell(t->x, c1), h2 = is_halo_cell(t->x, c2);
+    if (h1 && h2) return;
+    t->all += n1 * n2;
+    if ((!h1 && !h2) || (c1 < c2)) t->macro += n1 * n2;
+}
+
+int ox_count_checks(void *h, int n, const double *r, const double *q, const int *cid,
+                    long long *out) {
+    ox_ctx *x = (ox_ctx *)h;
+    int cap = n + 16 + n / 2;
+    ox_mol *mols = (ox_mol *)malloc(sizeof(ox_mol) * (size_t)cap);
+    if (!mols) return -1;
+    for (int i = 0; i < n; ++i) setup_mol(x, &mols[i], r + 3 * i, q + 4 * i, cid[i], i);
+    int total = n;
+    if (x->periodic) total = make_halo(x, &mols, n, &cap);
+    long ncell = (long)x->cpd[0] * x->cpd[1] * x->cpd[2];
+    int *cs = (int *)calloc((size_t)ncell + 1, sizeof(int));
+    for (int i = 0; i < total; ++i) cs[ox_cell_index_of_point(x, mols[i].r) + 1]++;
+    for (long c = 0; c < ncell; ++c) cs[c + 1] += cs[c];
+    count_state st = {x, cs, 0, 0};
+    walk_c08(x, count_cb, &st);
+    out[0] = st.all;
+    out[1] = st.macro;
     free(cs);
     free(mols);
     return 0;
@@ -1022,6 +1135,7 @@ static void qdiff(const double *q, const double *w, double *dq) {
 void ox_leapfrog_pre(void *h, int n, double dt, double *r, double *v, double *q, double *L,
                      const double *F, const double *M, const int *cid) {
     ox_ctx *x = (ox_ctx *)h;
+#pragma omp parallel for num_threads(x->nthreads) schedule(static) if (x->nthreads > 1)
     for (int i = 0; i < n; ++i) {
         const ox_comp *c = &x->comp[cid[i]];
         double dt_halve = .5 * dt;
@@ -1058,6 +1172,8 @@ void ox_leapfrog_post(void *h, int n, double dt, double *v, double *L, const dou
     ox_ctx *x = (ox_ctx *)h;
     double summv2 = 0, sumIw2 = 0, rot = 0;
     double dt_halve = 0.5 * dt;
+#pragma omp parallel for num_threads(x->nthreads) schedule(static) if (x->nthreads > 1) \
+    reduction(+ : summv2, sumIw2, rot)
     for (int i = 0; i < n; ++i) {
         const ox_comp *c = &x->comp[cid[i]];
         double dtInv2m = dt_halve / c->m;
@@ -1083,35 +1199,39 @@ void ox_leapfrog_post(void *h, int n, double dt, double *v, double *L, const dou
     sums[3] = rot;
 }
 
-/* <boundaries> per dimension (Simulation.cpp:429-444): 0 periodic, 2
- * reflecting (BoundaryUtils::BoundaryType order PERIODIC, OUTFLOW,
- * REFLECTING); outflow (1) is not restated. */
+/* <boundaries> per dimension (Simulation.cpp:429-444), in
+ * BoundaryUtils::BoundaryType order (BoundaryUtils.h:29): 0 periodic,
+ * 1 outflow, 2 reflecting. */
 int ox_set_boundaries(void *h, const int *types) {
     ox_ctx *x = (ox_ctx *)h;
     int any = 0;
     for (int d = 0; d < 3; ++d) {
-        if (types[d] != 0 && types[d] != 2) return -1;
+        if (types[d] < 0 || types[d] > 2) return -1;
         x->pdim[d] = types[d] == 0;
         x->wall[d] = types[d] == 2;
+        x->outflow[d] = types[d] == 1;
         any |= x->pdim[d];
     }
     x->periodic = any;
     return 0;
 }
 
-/* BoundaryHandler::processGlobalWallLeavingParticles for reflecting walls
- * (BoundaryHandler.cpp:82-134), walls in the order +x, -x, +y, -y, +z, -z:
- * a molecule inside the wall's slab of width rc (RegionUtils::
- * getInnerRegionSlab, RegionUtils.cpp:11-45; inBox l <= r < u) whose
- * predicted position r + dt (v + dt/2m F) reaches the wall
+/* BoundaryHandler::processGlobalWallLeavingParticles (BoundaryHandler.cpp:
+ * 82-134), walls in the map order of DimensionType (+x, -x, +y, -y, +z, -z,
+ * DimensionUtils.h:26): a molecule inside the wall's slab of width rc
+ * (RegionUtils::getInnerRegionSlab, RegionUtils.cpp:11-45; inBox l <= r < u)
+ * whose predicted position r + dt (v + dt/2m F) reaches the wall
  * (RegionUtils::isMoleculeLeaving, RegionUtils.cpp:47-56) gets
- * v = -v - 2 (dt/2m) F. */
+ * v = -v - 2 (dt/2m) F at a reflecting wall and is deleted at an outflow
+ * wall (dead[i] = 1; a deleted molecule is not seen by later walls). dead may
+ * be NULL when no wall is an outflow wall; molecules already dead on entry
+ * are skipped. */
 void ox_reflect(void *h, int n, double dt, const double *r, double *v, const double *F,
-                const int *cid) {
+                const int *cid, int *dead) {
     const ox_ctx *x = (const ox_ctx *)h;
     for (int wall = 0; wall < 6; ++wall) {
         const int d = wall / 2, pos = (wall % 2) == 0;
-        if (!x->wall[d]) continue;
+        if (!x->wall[d] && !(x->outflow[d] && dead)) continue;
         double lo[3], hi[3];
         for (int k = 0; k < 3; ++k) {
             lo[k] = x->bbmin[k];
@@ -1122,6 +1242,7 @@ void ox_reflect(void *h, int n, double dt, const double *r, double *v, const dou
         else
             hi[d] = fmin(x->bbmin[d] + x->rc, x->bbmax[d]);
         for (int i = 0; i < n; ++i) {
+            if (dead && dead[i]) continue;
             const double *ri = r + 3 * i;
             int in = 1;
             for (int k = 0; k < 3; ++k) in &= (ri[k] >= lo[k] && ri[k] < hi[k]);
@@ -1130,39 +1251,102 @@ void ox_reflect(void *h, int n, double dt, const double *r, double *v, const dou
             const double hm = half / x->comp[cid[i]].m;
             const double adj = hm * F[3 * i + d];
             const double np = ri[d] + (dt * (v[3 * i + d] + adj));
-            if ((pos && np >= x->bbmax[d]) || (!pos && np <= x->bbmin[d]))
-                v[3 * i + d] = -v[3 * i + d] - adj - adj;
+            if ((pos && np >= x->bbmax[d]) || (!pos && np <= x->bbmin[d])) {
+                if (x->wall[d])
+                    v[3 * i + d] = -v[3 * i + d] - adj - adj;
+                else
+                    dead[i] = 1;
+            }
         }
     }
 }
 
-static int any_wall(const ox_ctx *x) { return x->wall[0] | x->wall[1] | x->wall[2]; }
+static int any_wall(const ox_ctx *x) {
+    return x->wall[0] | x->wall[1] | x->wall[2] | x->outflow[0] | x->outflow[1] | x->outflow[2];
+}
+
+static int any_outflow(const ox_ctx *x) { return x->outflow[0] | x->outflow[1] | x->outflow[2]; }
+
+/* Removes the molecules flagged dead, keeping the order of the rest (the
+ * container's deleteMolecule, ParticleContainer.h:230); orig (may be NULL)
+ * moves along. Returns the new count. */
+static int compact_dead(int n, const int *dead, double *r, double *v, double *q, double *L,
+                        int *cid, double *F, double *M, double *Vi, int *orig) {
+    int k = 0;
+    for (int i = 0; i < n; ++i) {
+        if (dead[i]) continue;
+        if (k != i) {
+            for (int d = 0; d < 3; ++d) {
+                r[3 * k + d] = r[3 * i + d];
+                v[3 * k + d] = v[3 * i + d];
+                L[3 * k + d] = L[3 * i + d];
+                F[3 * k + d] = F[3 * i + d];
+                M[3 * k + d] = M[3 * i + d];
+                Vi[3 * k + d] = Vi[3 * i + d];
+            }
+            for (int d = 0; d < 4; ++d) q[4 * k + d] = q[4 * i + d];
+            cid[k] = cid[i];
+            if (orig) orig[k] = orig[i];
+        }
+        ++k;
+    }
+    return k;
+}
 
 /* Simulation::simulateOneTimestep step order (Simulation.cpp:1076-1307)
  * without thermostat, long-range correction or plugins: upd_preF, periodic
  * wrap, re-bin + halo + SoA, traversal, calcFM, upd_postF. F/M must hold the
  * forces of the initial configuration on entry (Simulation.cpp:909-925).
  * per_step[s*6 + ...] = Upot, virial, summv2, sumIw2, upot6lj/6, upotXpoles+myRF */
-int ox_simulate(void *h, int n, int nsteps, double dt, double *r, double *v, double *q,
-                double *L, const int *cid, double *F, double *M, double *Vi, double *per_step) {
+static int simulate_impl(void *h, int n, int nsteps, double dt, double *r, double *v, double *q,
+                         double *L, int *cid, double *F, double *M, double *Vi, int *orig,
+                         double *per_step, int width) {
     double mac[4], sums[4];
+    int *dead = any_outflow((ox_ctx *)h) ? (int *)calloc((size_t)n + 1, sizeof(int)) : NULL;
     for (int s = 0; s < nsteps; ++s) {
-        if (any_wall((ox_ctx *)h)) ox_reflect(h, n, dt, r, v, F, cid);
+        if (any_wall((ox_ctx *)h)) ox_reflect(h, n, dt, r, v, F, cid, dead);
+        if (dead) {  /* deleted before their upd_preF (Simulation.cpp:1114-1117) */
+            n = compact_dead(n, dead, r, v, q, L, cid, F, M, Vi, orig);
+            memset(dead, 0, sizeof(int) * (size_t)n);
+        }
         ox_leapfrog_pre(h, n, dt, r, v, q, L, F, M, cid);
         if (((ox_ctx *)h)->periodic) ox_wrap(h, n, r);
-        if (ox_forces(h, n, r, q, cid, F, M, Vi, mac, NULL)) return -1;
+        if (ox_forces(h, n, r, q, cid, F, M, Vi, mac, NULL)) {
+            free(dead);
+            return -1;
+        }
         ox_leapfrog_post(h, n, dt, v, L, q, F, M, cid, sums);
         if (per_step) {
-            double *o = per_step + 6 * s;
+            double *o = per_step + width * s;
             o[0] = mac[OX_UPOT6LJ] / 6.0 + mac[OX_UPOTXPOLES] + mac[OX_MYRF];
             o[1] = mac[OX_VIRIAL] + 3.0 * mac[OX_MYRF];
             o[2] = sums[0];
             o[3] = sums[1];
             o[4] = mac[OX_UPOT6LJ] / 6.0;
             o[5] = mac[OX_UPOTXPOLES] + mac[OX_MYRF];
+            if (width > 6) o[6] = sums[2];
         }
     }
-    return 0;
+    free(dead);
+    return n;
+}
+
+int ox_simulate(void *h, int n, int nsteps, double dt, double *r, double *v, double *q,
+                double *L, const int *cid, double *F, double *M, double *Vi, double *per_step) {
+    if (any_outflow((ox_ctx *)h)) return -1; /* the count would change: ox_simulate_open */
+    return simulate_impl(h, n, nsteps, dt, r, v, q, L, (int *)cid, F, M, Vi, NULL, per_step, 6) < 0
+               ? -1
+               : 0;
+}
+
+/* ox_simulate with outflow walls: molecules leaving through them are
+ * deleted, the arrays are compacted in place (survivors in their order, orig
+ * carries each survivor's index on entry) and the count is returned;
+ * per_step rows get a 7th column, N. */
+int ox_simulate_open(void *h, int n, int nsteps, double dt, double *r, double *v, double *q,
+                     double *L, int *cid, double *F, double *M, double *Vi, int *orig,
+                     double *per_step) {
+    return simulate_impl(h, n, nsteps, dt, r, v, q, L, cid, F, M, Vi, orig, per_step, 7);
 }
 
 /* ------------------------------------------- global values, LRC, thermostat */
@@ -1356,8 +1540,9 @@ int ox_simulate_nvt(void *h, int n, int nsteps, double dt, double *r, double *v,
                     double *L, const int *cid, double *F, double *M, double *Vi,
                     const double *lrc, double Ttarget, int heuristics, double *per_step) {
     double mac[4], sums[4], tp[4];
+    if (any_outflow((ox_ctx *)h)) return -1; /* NVT runs keep their molecule count */
     for (int s = 0; s < nsteps; ++s) {
-        if (any_wall((ox_ctx *)h)) ox_reflect(h, n, dt, r, v, F, cid);
+        if (any_wall((ox_ctx *)h)) ox_reflect(h, n, dt, r, v, F, cid, NULL);
         ox_leapfrog_pre(h, n, dt, r, v, q, L, F, M, cid);
         if (((ox_ctx *)h)->periodic) ox_wrap(h, n, r);
         if (ox_forces(h, n, r, q, cid, F, M, Vi, mac, NULL)) return -1;

@@ -36,6 +36,14 @@ void *ox_create(int ncomp, const double *desc, int desc_len, const double *mix,
                 double epsRF, double rc, double rcLJ, const double *box,
                 int periodic, int cell_mode);
 void ox_destroy(void *h);
+/* OpenMP threads of the force traversal and the molecule loops (default 1,
+ * the serial path every test uses; more only for timing the CPU baseline). */
+void ox_set_threads(void *h, int nthreads);
+/* Molecule-centre distance checks of the reference's C08 traversal: out[0]
+ * all (halo pairs of periodic boundaries twice), out[1] those with
+ * macroscopic sums (each unordered pair once). */
+int ox_count_checks(void *h, int n, const double *r, const double *q, const int *cid,
+                    long long *out);
 
 int ox_nlj_total(void *h);
 void ox_lj_tables(void *h, double *eps24, double *sig2, double *shift6);
@@ -68,7 +76,10 @@ int ox_simulate(void *h, int n, int nsteps, double dt, double *r, double *v,
  * walls' velocity reversal; see ls1_oracle.c */
 int ox_set_boundaries(void *h, const int *types);
 void ox_reflect(void *h, int n, double dt, const double *r, double *v, const double *F,
-                const int *cid);
+                const int *cid, int *dead);
+int ox_simulate_open(void *h, int n, int nsteps, double dt, double *r, double *v, double *q,
+                     double *L, int *cid, double *F, double *M, double *Vi, int *orig,
+                     double *per_step);
 
 /* global values, homogeneous long-range correction, velocity-scaling
  * thermostat; see ls1_oracle.c */

@@ -70,7 +70,10 @@ class Oracle:
             L.ox_velocity_scaling.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip,
                                               _dp, C.c_double]
             L.ox_set_boundaries.argtypes = [C.c_void_p, _ip]
-            L.ox_reflect.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp, _dp, _dp, _ip]
+            L.ox_reflect.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp, _dp, _dp, _ip,
+                                     C.c_void_p]
+            L.ox_simulate_open.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp,
+                                           _dp, _dp, _ip, _dp, _dp, _dp, _ip, _dp]
             L.ox_simulate_rmm.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp,
                                           _ip, _dp]
             L.ox_simulate_nvt.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp,
@@ -79,6 +82,8 @@ class Oracle:
             L.ox_lj_shift_rc.argtypes = [C.c_void_p, _dp]
             L.ox_adjoint.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp,
                                      _dp, _dp, _dp]
+            L.ox_set_threads.argtypes = [C.c_void_p, C.c_int]
+            L.ox_count_checks.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _ip, _lp]
             L.ox_site_pair.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp, C.c_double,
                                        C.c_double, C.c_double, C.c_double, C.c_double, _dp]
             cls._lib = L
@@ -107,6 +112,20 @@ class Oracle:
     def from_phase_space(cls, ps, rc, rc_lj=None, periodic=True, cell_mode=0):
         return cls(ps.ncomp, ps.comp_desc, ps.mix, ps.eps_rf, rc, rc if rc_lj is None else rc_lj,
                    ps.box, periodic, cell_mode)
+
+    def set_threads(self, n):
+        """OpenMP threads of the traversal and molecule loops (1: the serial
+        path every parity test uses)."""
+        self.lib().ox_set_threads(self.h, int(n))
+
+    def count_checks(self, r, q, cid):
+        """Molecule-centre distance checks of the reference's C08 traversal:
+        (all, those with macroscopic sums)."""
+        out = np.zeros(2, np.int64)
+        if self.lib().ox_count_checks(self.h, len(r), _f64(r), _f64(q),
+                                      np.ascontiguousarray(cid, np.int32), out):
+            raise RuntimeError("oracle count failed")
+        return int(out[0]), int(out[1])
 
     def lj_tables(self):
         n = self.lib().ox_nlj_total(self.h)
@@ -255,15 +274,36 @@ class Oracle:
                     lrc=terms)
 
     def set_boundaries(self, types):
-        """Per dimension: 0 periodic, 2 reflecting."""
+        """Per dimension: 0 periodic, 1 outflow, 2 reflecting."""
         if self.lib().ox_set_boundaries(self.h, np.ascontiguousarray(types, np.int32)):
             raise ValueError("unsupported boundary type")
 
-    def reflect(self, dt, r, v, F, cid):
+    def reflect(self, dt, r, v, F, cid, with_dead=False):
+        """The wall rule: new velocities (reflecting walls) and, with
+        with_dead, the deletion flags of outflow walls."""
         v = _f64(v).copy()
+        dead = np.zeros(len(v), np.int32)
         self.lib().ox_reflect(self.h, len(v), float(dt), _f64(r), v, _f64(F),
-                              np.ascontiguousarray(cid, np.int32))
-        return v
+                              np.ascontiguousarray(cid, np.int32),
+                              dead.ctypes.data_as(C.c_void_p) if with_dead else None)
+        return (v, dead.astype(bool)) if with_dead else v
+
+    def simulate_open(self, nsteps, dt, r, v, q, L, cid):
+        """simulate with outflow walls: survivors (compacted, orig = their
+        upload index) and per-step rows with a 7th column N."""
+        r, v, q, L = (_f64(a).copy() for a in (r, v, q, L))
+        cid = np.ascontiguousarray(cid, np.int32).copy()
+        f0 = self.forces(r, q, cid)
+        q = f0["q"]
+        F, M, Vi = f0["F"].copy(), f0["M"].copy(), f0["Vi"].copy()
+        orig = np.arange(len(r), dtype=np.int32)
+        per = np.zeros(7 * max(nsteps, 1))
+        k = self.lib().ox_simulate_open(self.h, len(r), int(nsteps), float(dt), r, v, q, L, cid,
+                                        F, M, Vi, orig, per)
+        if k < 0:
+            raise RuntimeError("oracle simulate_open failed")
+        return dict(r=r[:k], v=v[:k], q=q[:k], L=L[:k], F=F[:k], M=M[:k], Vi=Vi[:k], cid=cid[:k],
+                    orig=orig[:k], per_step=per.reshape(-1, 7)[:nsteps])
 
     def simulate_rmm(self, nsteps, dt, r, v, q, cid):
         """Reduced-memory mode: initial half kick, then nsteps; returns the

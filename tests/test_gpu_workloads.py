@@ -48,6 +48,9 @@ def gpu_eval(s, precision, rc_lj=None, cell_mode=0, steps=0):
         ctx.set_counting(True)
         out = ctx.forces()
         out["counters"] = ctx.counters()
+        # per-kernel visit counts: each kernel its own side, halved in total
+        kc = ctx.kernel_stats()[1]
+        np.testing.assert_array_equal(kc.sum(0) // 2, out["counters"])
         ctx.set_counting(False)
         if steps:
             out["per_step"] = ctx.step(steps, s.dt, use_graph=True)
@@ -191,7 +194,7 @@ def test_full_replica_sweep_properties():
 
 # --------------------------------------------------- split cutoffs rcLJ < rc
 
-SPLIT_FIXTURES = [("VectorizationWater", 6.16), ("VectorizationMultiComponentMultiPotentials", 35.0)]
+SPLIT_FIXTURES = [("VectorizationWater", 12.0), ("VectorizationMultiComponentMultiPotentials", 35.0)]
 
 
 @pytest.mark.parametrize("name,rc", SPLIT_FIXTURES)

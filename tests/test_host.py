@@ -43,7 +43,7 @@ def pressure(per_step, n, volume, ndof):
 
 
 def parsed(stdout, key):
-    return np.array([float(m) for m in re.findall(key + r" = ([-0-9.eE+]+)", stdout)])
+    return np.array([float(m) for m in re.findall(r"\b" + key + r" = ([-0-9.eE+]+)", stdout)])
 
 
 @pytest.mark.gpu

@@ -296,3 +296,34 @@ def test_energy_conservation_short_run():
     ps = res["per_step"]
     e = ps[:, 0] + 0.5 * (ps[:, 2] + ps[:, 3])
     assert np.abs(e - e[0]).max() / abs(e[0]) < 1e-4
+
+
+def test_threaded_traversal_matches_serial():
+    """The OpenMP C08 walk (CPU baseline timing only) gives the serial walk's
+    forces to rounding of the macroscopic sums' order, and equal counters."""
+    s = scenarios.small_mixture(n=600, seed=29)
+    a = Oracle(s.ncomp, s.comp_desc, s.mix, s.eps_rf, s.rc, s.rc_lj, s.box).forces(s.r, s.q, s.cid)
+    o = Oracle(s.ncomp, s.comp_desc, s.mix, s.eps_rf, s.rc, s.rc_lj, s.box)
+    o.set_threads(4)
+    b = o.forces(s.r, s.q, s.cid)
+    for k in ("F", "M", "Vi"):
+        np.testing.assert_array_equal(a[k], b[k])  # per-molecule sums keep their order
+    np.testing.assert_allclose(b["macro"], a["macro"], rtol=1e-13)
+    np.testing.assert_array_equal(a["counters"], b["counters"])
+
+
+def test_reference_check_count():
+    """ox_count_checks: the checks with macroscopic sums equal the traversal's
+    own molecule-pair counter; all checks add the boundary-halo pairs seen
+    from the second periodic side (none in an open system)."""
+    s = scenarios.small_lj(n=500, seed=31)
+    o = Oracle(1, s.comp_desc, None, 1.0, s.rc, s.rc, s.box)
+    ref = o.forces(s.r, s.q, s.cid)
+    allc, macro = o.count_checks(s.r, s.q, s.cid)
+    assert macro == ref["counters"][0] and allc > macro
+    n = s.n
+    # brute force over minimum images: unordered pairs whose cells are neighbours
+    # are all checked once with macroscopic sums
+    op = Oracle(1, s.comp_desc, None, 1.0, s.rc, s.rc, s.box, periodic=False)
+    a2, m2 = op.count_checks(s.r, s.q, s.cid)
+    assert a2 == m2 and m2 <= n * (n - 1) // 2

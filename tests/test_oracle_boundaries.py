@@ -69,3 +69,70 @@ def test_reflecting_box_keeps_molecules_inside():
     o.set_boundaries((REF, REF, REF))
     res = o.simulate(200, 0.002, s.r, s.v, s.q, s.L, s.cid)
     assert np.all(res["r"] >= 0.0) and np.all(res["r"] < s.box)
+
+
+OUT = 1
+
+
+def test_outflow_rule():
+    """Outflow walls delete instead of reflect (BoundaryHandler.cpp:95-130):
+    same slab and leaving test as the reflecting rule, hand-built cases."""
+    box = np.array([10.0, 10.0, 10.0])
+    desc = np.array(scenarios.comp_1clj(m=2.0), np.float64)
+    o = Oracle(1, desc, None, 1.0, 2.5, 2.5, box)
+    o.set_boundaries((PER, PER, OUT))
+    dt = 0.01
+    r = np.array([[5, 5, 9.99], [5, 5, 9.99], [5, 5, 6.0], [5, 5, 0.005], [5, 5, 9.99],
+                  [5, 5, 7.4], [5, 5, 7.5]])
+    v = np.array([[0, 0, 2.0], [0, 0, -2.0], [0, 0, 500.0], [0, 0, -1.0], [0, 0, 0.1],
+                  [0, 0, 300.0], [0, 0, 300.0]])
+    F = np.array([[0, 0, 3.0], [0, 0, 1.0], [0, 0, 0.0], [0, 0, 0.5], [0, 0, 400.0],
+                  [0, 0, 0.0], [0, 0, 0.0]])
+    cid = np.zeros(len(r), np.int32)
+    v2, dead = o.reflect(dt, r, v, F, cid, with_dead=True)
+    # leaving through +z (0, 4: the half kick carries it out, 6: at the slab's
+    # lower edge 7.5 = 10 - rc, which belongs to the slab) or -z (3)
+    np.testing.assert_array_equal(dead, [True, False, False, True, True, False, True])
+    np.testing.assert_array_equal(v2, v)  # outflow never changes velocities
+
+
+def test_outflow_and_reflecting_corner():
+    """A molecule leaving through a reflecting x wall and an outflow z wall is
+    deleted; one leaving through the reflecting wall only is reflected."""
+    box = np.array([10.0, 10.0, 10.0])
+    desc = np.array(scenarios.comp_1clj(), np.float64)
+    o = Oracle(1, desc, None, 1.0, 2.5, 2.5, box)
+    o.set_boundaries((REF, PER, OUT))
+    dt = 0.01
+    r = np.array([[9.99, 5, 9.99], [9.99, 5, 5.0]])
+    v = np.array([[2.0, 0, 2.0], [2.0, 0, 2.0]])
+    F = np.zeros((2, 3))
+    v2, dead = o.reflect(dt, r, v, F, np.zeros(2, np.int32), with_dead=True)
+    np.testing.assert_array_equal(dead, [True, False])
+    assert v2[1, 0] == -2.0 and v2[1, 2] == 2.0
+
+
+def test_outflow_simulation_deletes_leavers():
+    """Isolated molecules (no partner within rc) streaming towards the +x
+    outflow wall: each is deleted in the step whose wall test it fails, the
+    others drift on unchanged; survivors keep their upload order."""
+    box = np.array([12.0, 12.0, 12.0])
+    desc = np.array(scenarios.comp_1clj(), np.float64)
+    o = Oracle(1, desc, None, 1.0, 2.5, 2.5, box)
+    o.set_boundaries((OUT, PER, PER))
+    dt = 0.0078125  # drifts of whole quarters: every position below is exact
+    # x positions and x velocities; y, z spread so no pair is within rc
+    x = np.array([11.5, 11.0, 9.75, 6.0, 0.5])
+    vx = np.array([128.0, 32.0, 64.0, 0.0, -32.0])
+    n = len(x)
+    r = np.stack([x, 3.0 * np.arange(n) % 12.0 + 0.5, (np.arange(n) * 2.9) % 12.0], axis=1)
+    v = np.stack([vx, np.zeros(n), np.zeros(n)], axis=1)
+    q = np.tile([1.0, 0, 0, 0], (n, 1))
+    res = o.simulate_open(10, dt, r, v, q, np.zeros((n, 3)), np.zeros(n, np.int32))
+    # by hand, the test before each step's drift (x + dt v against the wall,
+    # inside the slab of width rc): molecule 0 at step 1 (11.5 + 1 >= 12),
+    # molecule 4 at step 2 (0.25 - 0.25 <= 0), molecule 1 at step 4 (11.75 +
+    # 0.25 >= 12), molecule 2 at step 5 (11.75 + 0.5 >= 12)
+    np.testing.assert_array_equal(res["orig"], [3])
+    np.testing.assert_array_equal(res["per_step"][:, 6], [4, 3, 3, 2, 1, 1, 1, 1, 1, 1])
+    np.testing.assert_allclose(res["r"][0], r[3], rtol=0, atol=0)
