@@ -166,23 +166,34 @@ def run_gpu(s, precision, steps, warmup, kernel_reps, rmm=False):
                 kernel_counters=kcounters, kernel_ms=np.mean(np.array(kernel_ms), axis=0))
 
 
-def kernel_rooflines(r, tag, workload, scale):
-    """One roofline entry per pair kernel that ran: its FlopCounter flops (its
-    share of the pairs: each kernel evaluates its own side, so a pair of one
-    type counts half in its kernel's visits twice and a mixed pair half in
-    each of the two kernels) over its mean HIP-event duration."""
+def kernel_rooflines(r, tag, workload, scale, ref_checks):
+    """One roofline entry per pair kernel that ran, on the FlopCounter basis
+    (FlopCounter.cpp:19-76): the site pairs the kernel evaluates (its share:
+    every site evaluates its own side, so a pair of one type counts half in
+    its kernel's visits twice and a mixed pair half in each of the two
+    kernels) at the per-type flops, plus 8 flops per molecule-centre check,
+    with ls1's count of checks (each unordered molecule pair once, from the
+    oracle's C08 walk) shared among the kernels in proportion to the checks
+    each makes; the kernels' flops then add up to the FlopCounter total of a
+    traversal. flops_issued_basis counts each kernel's own checks instead (a
+    multi-site kernel checks every site pair's molecule centres)."""
     peak = PEAK_FP32_VECTOR_TF if tag == "fp32" else PEAK_FP64_VECTOR_TF
+    kc = np.asarray(r["kernel_counters"], np.float64) / 2.0
+    own_checks = kc[:, 0].sum()
     out = []
     for t, name in enumerate(KERNEL_NAMES):
         ms = float(r["kernel_ms"][t])
         if ms <= 0:
             continue
-        cnt = np.asarray(r["kernel_counters"][t], np.float64) / 2.0
-        flops = float((FLOPS_PER_PAIR * cnt).sum())
+        cnt = kc[t]
+        site_flops = float((FLOPS_PER_PAIR[1:] * cnt[1:]).sum())
+        share = cnt[0] / own_checks if own_checks > 0 else 0.0
+        flops = site_flops + 8.0 * ref_checks * share
         achieved = flops / (ms * 1e-3) / 1e12
         out.append({"kernel": name, "bound": "valu", "achieved": achieved, "peak": peak,
                     "unit": "TFLOP/s", "frac": achieved / peak, "flops_per_launch": flops,
-                    "kernel_ms": ms, "molecule_checks": float(cnt[0]),
+                    "flops_issued_basis": site_flops + 8.0 * float(cnt[0]),
+                    "kernel_ms": ms, "molecule_checks_made": float(cnt[0]),
                     "site_pairs": {k: float(v) for k, v in zip(
                         ("LJ", "CC", "CD", "DD", "CQ", "DQ", "QQ"), cnt[1:]) if v},
                     "traffic": measured_traffic(workload, scale, tag, r["shape"], name)})
@@ -352,10 +363,26 @@ def run_adjoint(args):
         for _ in range(args.steps):
             ctx.adjoint(None)
         adj_p = (time.perf_counter() - t0) / args.steps
+        # kernel times (HIP events on the context stream, transfers excluded)
+        ctx.set_timing(True)
+        fk, ak, vk = [], [], []
+        for _ in range(args.steps):
+            ctx.traverse()
+            fk.append(ctx.last_kernel_ms()[0])
+            ctx.adjoint(cot)
+            a_ms, v_ms = ctx.last_kernel_ms()
+            ak.append(a_ms)
+            vk.append(v_ms)
+        ctx.set_timing(False)
+    fwd_k = float(np.mean(fk))
     return {"workload": "binary_lj adjoint", "molecules": int(s.n),
-            "forward_traverse_ms": fwd * 1e3, "adjoint_with_vjp_ms": adj * 1e3,
-            "adjoint_params_only_ms": adj_p * 1e3,
-            "note": "host-call wall time incl. cotangent upload and result download"}
+            "forward_pair_kernel_ms": fwd_k, "adjoint_kernels_ms": float(np.mean(ak)),
+            "adjoint_kernels_with_vjp_ms": float(np.mean(vk)),
+            "adjoint_over_forward": float(np.mean(vk)) / fwd_k,
+            "host_call_ms": {"forward_traverse": fwd * 1e3, "adjoint_with_vjp": adj * 1e3,
+                             "adjoint_params_only": adj_p * 1e3},
+            "note": "kernel times from HIP events; host-call times include the cotangent "
+                    "upload and the result download (pageable host memory)"}
 
 
 def rel_rms(a, b):
@@ -517,9 +544,11 @@ def measure(workload, scale, args, quiet=False):
     r = res[main_tag]
     ms = r["seconds"] * 1e3 / args.steps
     value = s.n * args.steps / r["seconds"]
-    kern = {tag: kernel_rooflines(res[tag], tag, workload, scale) for tag in res}
+    log("reference-equivalent molecule-centre checks (oracle counting walk)")
+    ref_all, ref_macro = reference_checks(s)
+    kern = {tag: kernel_rooflines(res[tag], tag, workload, scale, ref_macro) for tag in res}
     top = max(kern[main_tag], key=lambda k: k["kernel_ms"])
-    all_flops = float((FLOPS_PER_PAIR * r["counters"]).sum())
+    all_flops = float(8.0 * ref_macro + (FLOPS_PER_PAIR[1:] * r["counters"][1:]).sum())
     potential = {"ljts_slab": "LJTS 12-6", "mixture": "2CLJQ + 4-site water (LJ, charge, "
                  "quadrupole)", "lj_fcc": "LJ 12-6", "binary_lj": "binary LJ 12-6"}.get(workload,
                                                                                       workload)
@@ -544,6 +573,7 @@ def measure(workload, scale, args, quiet=False):
         "kernels": kern,
         "pair_kernels_ms": r["pair_ms"],
         "pair_kernels_flops_per_launch": all_flops,
+        "pair_kernels_achieved_tflops": all_flops / (r["pair_ms"] * 1e-3) / 1e12,
         "launch_shape": {"cells_per_block": r["shape"][0], "blocks_per_xcd": r["shape"][1]},
     }
     if "fp64" in res and "fp32" in res:
@@ -569,15 +599,14 @@ def measure(workload, scale, args, quiet=False):
             out.setdefault("reduced_memory_mode", {})[tag] = {
                 "value": s.n * args.steps / rs, "ms_per_step": rs * 1e3 / args.steps,
                 "note": "VCP1CLJRMM + LeapfrogRMM equivalent: kick inside the pair kernel"}
-    log("reference-equivalent molecule-centre checks (oracle counting walk)")
-    ref_all, ref_macro = reference_checks(s)
     out["molecule_checks"] = {
         "this_build": float(r["counters"][0]),
         "reference_c08_all": float(ref_all),
         "reference_c08_with_macroscopic_sums": float(ref_macro),
-        "note": "the flop count uses this build's checks (each unordered pair once, charge "
-                "partners tested once per molecule); ls1's FlopCounter counts every check its "
-                "C08 traversal makes, including the second visit of boundary-halo pairs"}
+        "note": "flops use ls1's checks with macroscopic sums (each unordered molecule pair "
+                "once); this build checks molecule centres once per pair of sites of the "
+                "kernel's types (charge partners once per molecule), ls1's FlopCounter also "
+                "counts the second visit of boundary-halo pairs (reference_c08_all)"}
     return out
 
 
