@@ -318,6 +318,22 @@ int mdx_adjoint(mdx_context *ctx, int64_t n, const double *cot, double *dU_deps,
                 double *dU_dsig, double *dW_deps, double *dW_dsig, double *dU_dq,
                 double *dW_dq, double *vjp);
 
+/* mdx_adjoint without any allocation. No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight fitting or serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, checks and results as mdx_adjoint, but its device buffers
+ * (parameter partials, reduced tables, per-site VJP terms, the reordered
+ * cotangent) live in a 64 MiB scratch array that the library holds once per
+ * device until the process ends, instead of buffers mdx_adjoint allocates on
+ * its first call. A state whose buffers do not fit is refused with
+ * MDX_ERR_CAPACITY (about 1.3 M one-centre molecules with a cotangent; any
+ * size without one). Every call on a device shares that device's array:
+ * calls that overlap on one device (several contexts driven from several
+ * host threads) race and must be serialised by the caller. */
+int mdx_adjoint_noalloc(mdx_context *ctx, int64_t n, const double *cot, double *dU_deps,
+                        double *dU_dsig, double *dW_deps, double *dW_dsig, double *dU_dq,
+                        double *dW_dq, double *vjp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -57,8 +58,19 @@ thread_local std::string g_create_error;
 constexpr int kBlock = 256;
 inline int nblk(long n, int b = kBlock) { return (int)std::max<long>(1, (n + b - 1) / b); }
 
+// Scratch of the entry points that allocate nothing (mdx_adjoint_noalloc):
+// 64 MiB per device, held from the library's first use on a device until the
+// process ends. Host code never uses the array's name as an address: each
+// device's copy is resolved once, by mdx_create on that device, with
+// hipGetSymbolAddress, and the resolved pointer is what kernels receive.
+constexpr size_t kScratchDoubles = (size_t)8 << 20;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
 }  // namespace
 
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 struct mdx_context {
     int device = 0;
@@ -127,6 +139,7 @@ struct mdx_context {
     double *red_tmp = nullptr;  // first level of the two-level reductions
     double *dl_tmp = nullptr;   // download staging (4 n)
     double *adj_site = nullptr, *adj_cot = nullptr, *adj_part = nullptr, *adj_out = nullptr;
+    double *scratch = nullptr;  // this device's copy of the library scratch array
     int *nsite_mol[kTypes] = {nullptr, nullptr, nullptr, nullptr};
     int *ident = nullptr;  // 0 .. n (point_lj: molecule index of each site, site starts)
     SiteArrays sites[kTypes]{};
@@ -1090,6 +1103,19 @@ int mdx_create(int device, mdx_context **out) {
     }
     (void)hipMemset(c->step_idx, 0, sizeof(int));
     (void)hipMemset(c->d_counters, 0, 8 * kTypes * sizeof(unsigned long long));
+    if (device < kMaxDevices) {
+        // once per device, with that device current
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[device] && g_scratch_err[device] == hipSuccess) {
+            void *p = nullptr;
+            g_scratch_err[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (g_scratch_err[device] == hipSuccess)
+                g_scratch_ptr[device] = (double *)p;
+            else
+                (void)hipGetLastError();  // keep the failure out of later launch checks
+        }
+        c->scratch = g_scratch_ptr[device];
+    }
     *out = c;
     return MDX_OK;
 }
@@ -2002,6 +2028,29 @@ int mdx_adjoint(mdx_context *c, int64_t n, const double *cot, double *dU_deps, d
     }
     return adjoint_run(c, cot, dU_deps, dU_dsig, dW_deps, dW_dsig, dU_dq, dW_dq, vjp,
                        AdjBufs{c->adj_site, c->adj_cot, c->adj_part, c->adj_out});
+}
+
+int mdx_adjoint_noalloc(mdx_context *c, int64_t n, const double *cot, double *dU_deps,
+                        double *dU_dsig, double *dW_deps, double *dW_dsig, double *dU_dq,
+                        double *dW_dq, double *vjp) {
+    TRY(adjoint_check(c, n, cot, vjp));
+    if (!c->scratch)
+        return fail(c, MDX_ERR_HIP, "the scratch array of this device could not be resolved");
+    // [out | part | site | cot], each part 256-byte aligned (32 doubles)
+    auto up = [](size_t x) { return (x + 31) & ~(size_t)31; };
+    const size_t o_part = up(kAdjOutDoubles);
+    const size_t o_site = o_part + up((size_t)adjoint_grid(c) * adjoint_table_width(c));
+    const size_t nsite = cot ? 3 * ((size_t)c->nsites[T_LJ] + c->nsites[T_C]) : 0;
+    const size_t o_cot = o_site + up(nsite);
+    const size_t total = o_cot + (cot ? 3 * (size_t)c->n : 0);
+    if (total > kScratchDoubles)
+        return fail(c, MDX_ERR_CAPACITY,
+                    "mdx_adjoint_noalloc: the state needs " + std::to_string(total * 8) +
+                        " bytes of scratch, more than the " + std::to_string(kScratchDoubles * 8) +
+                        " bytes the library holds per device (use mdx_adjoint)");
+    double *s = c->scratch;
+    return adjoint_run(c, cot, dU_deps, dU_dsig, dW_deps, dW_dsig, dU_dq, dW_dq, vjp,
+                       AdjBufs{s + o_site, s + o_cot, s + o_part, s});
 }
 
 int mdx_set_counting(mdx_context *c, int enable) {

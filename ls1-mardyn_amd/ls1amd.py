@@ -32,7 +32,7 @@ EXPORTS = (
     "mdx_upload_device", "mdx_set_timing", "mdx_last_kernel_ms", "mdx_set_charges",
     "mdx_adjoint", "mdx_lj_tables", "mdx_set_replicas", "mdx_set_pair_config", "mdx_autotune",
     "mdx_set_thermostat", "mdx_thermostat_state", "mdx_set_rmm", "mdx_set_boundaries",
-    "mdx_molecule_count", "mdx_kernel_stats",
+    "mdx_adjoint_noalloc", "mdx_molecule_count", "mdx_kernel_stats",
 )
 
 _lib = None
@@ -85,6 +85,7 @@ def load():
     L.mdx_set_rmm.argtypes = [vp, C.c_double]
     L.mdx_set_boundaries.argtypes = [vp, _ip]
     L.mdx_adjoint.argtypes = [vp, C.c_int64] + [vp] * 8
+    L.mdx_adjoint_noalloc.argtypes = [vp, C.c_int64] + [vp] * 8
     L.mdx_molecule_count.argtypes = [vp, vp]
     L.mdx_kernel_stats.argtypes = [vp, vp, vp]
     _lib = L
@@ -353,11 +354,13 @@ class Context:
         q = np.ascontiguousarray(q, np.float64)
         self._chk(self.L.mdx_set_charges(self.h, q.shape[0], q.ravel()), "mdx_set_charges")
 
-    def adjoint(self, cot=None):
+    def adjoint(self, cot=None, noalloc=False):
         """FP64 adjoint for the state of the last update_container:
         dU/dW with respect to LJ pair eps/sigma ([nlj, nlj]) and charges
         ([ncomp, 8]); with a cotangent on the molecule forces, the force
-        vector-Jacobian product with respect to positions ([n, 3])."""
+        vector-Jacobian product with respect to positions ([n, 3]).
+        noalloc: through mdx_adjoint_noalloc (the library's per-device
+        scratch array instead of buffers allocated on the first call)."""
         n, nl, ncomp = self.n, self.nlj, self.ncomp
         out = {k: np.zeros((nl, nl)) for k in ("dU_deps", "dU_dsig", "dW_deps", "dW_dsig")}
         out["dU_dq"], out["dW_dq"] = np.zeros((ncomp, 8)), np.zeros((ncomp, 8))
@@ -365,7 +368,8 @@ class Context:
         if cot is not None:
             c = np.ascontiguousarray(cot, np.float64).reshape(n, 3)
             out["vjp"] = np.zeros((n, 3))
-        fn, name = self.L.mdx_adjoint, "mdx_adjoint"
+        fn, name = ((self.L.mdx_adjoint_noalloc, "mdx_adjoint_noalloc") if noalloc
+                    else (self.L.mdx_adjoint, "mdx_adjoint"))
         self._chk(fn(self.h, n, _ptr(c), *(_ptr(out[k]) for k in (
             "dU_deps", "dU_dsig", "dW_deps", "dW_dsig", "dU_dq", "dW_dq")), _ptr(out.get("vjp"))),
             name)
