@@ -111,6 +111,8 @@ struct mdx_context {
     std::vector<double> shift_rc;  // radius of the shift tied to (eps, sigma); 0 = none
     double *d_eps24 = nullptr, *d_sig2 = nullptr, *d_shift6 = nullptr, *d_shift_rc = nullptr;
     float *d_feps24 = nullptr, *d_fsig2 = nullptr, *d_fshift6 = nullptr;
+    double *d_dpar = nullptr;  // interleaved {eps24, sig2, shift6, 0} per id pair
+    float *d_fpar = nullptr;
     double eps_rf = 1.0;
     // domain: nrep replicas (1 for a single system; g = replica 0)
     Geom g{};
@@ -370,12 +372,21 @@ void build_tables(mdx_context *c) {
 int upload_tables(mdx_context *c) {
     size_t nn = c->eps24.size();
     if (nn == 0) return MDX_OK;
-    std::vector<float> fe(nn), fs(nn), fh(nn);
+    std::vector<float> fe(nn), fs(nn), fh(nn), fpar(4 * nn, 0.f);
+    std::vector<double> dpar(4 * nn, 0.0);
     for (size_t i = 0; i < nn; ++i) {
         fe[i] = (float)c->eps24[i];
         fs[i] = (float)c->sig2[i];
         fh[i] = (float)c->shift6[i];
+        dpar[4 * i] = c->eps24[i];
+        dpar[4 * i + 1] = c->sig2[i];
+        dpar[4 * i + 2] = c->shift6[i];
+        fpar[4 * i] = fe[i];
+        fpar[4 * i + 1] = fs[i];
+        fpar[4 * i + 2] = fh[i];
     }
+    HIPCHK(c, hipMemcpy(c->d_dpar, dpar.data(), 4 * nn * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_fpar, fpar.data(), 4 * nn * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_eps24, c->eps24.data(), nn * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_sig2, c->sig2.data(), nn * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_shift6, c->shift6.data(), nn * 8, hipMemcpyHostToDevice));
@@ -814,6 +825,8 @@ int launch_traverse(mdx_context *c, double kick = 0.0) {
     a.pp.feps24 = c->d_feps24;
     a.pp.fsig2 = c->d_fsig2;
     a.pp.fshift6 = c->d_fshift6;
+    a.pp.dpar = c->d_dpar;
+    a.pp.fpar = c->d_fpar;
     a.pp.nlj = c->model.nlj;
     a.ncell = c->ncell;
     a.cpb = c->cpb;
@@ -1156,6 +1169,8 @@ int mdx_set_components(mdx_context *c, int ncomp, const double *desc, int desc_l
     TRY(dalloc(c, &c->d_feps24, nn));
     TRY(dalloc(c, &c->d_fsig2, nn));
     TRY(dalloc(c, &c->d_fshift6, nn));
+    TRY(dalloc(c, &c->d_dpar, 4 * nn));
+    TRY(dalloc(c, &c->d_fpar, 4 * nn));
     c->have_model = true;
     c->have_domain = false;
     return MDX_OK;

@@ -84,6 +84,11 @@ struct MolArrays {
 struct PairParams {
     const double *eps24, *sig2, *shift6;  // nlj x nlj
     const float *feps24, *fsig2, *fshift6;
+    // the same tables interleaved, one 4-value record {eps24, sig2, shift6, 0}
+    // per pair of LJ ids: a pair kernel fetches a partner's parameters with
+    // one (FP32) or two (FP64) 16-byte loads at a 32-bit offset
+    const double *dpar;
+    const float *fpar;
     int nlj;
 };
 

@@ -31,10 +31,22 @@ template <> struct Math<double> {
         return __builtin_fma(y, e, y);
     }
     static __device__ __forceinline__ double rinv(double r2, double r2inv) { return sqrt(r2inv); }
+    // 1/r^2 and 1/r of one distance
+    static __device__ __forceinline__ void inv2(double r2, double &r2inv, double &rinv) {
+        r2inv = rcp(r2);
+        rinv = sqrt(r2inv);
+    }
 };
 template <> struct Math<float> {
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ float rinv(float r2, float) { return __builtin_amdgcn_rsqf(r2); }
+    // two hardware approximations (1 ulp each); 1/r^2 as the square of 1/r
+    // would save one but doubles the bias of v_rsq_f32 in every force, which
+    // the sums of neutral molecules (charges of both signs) amplify
+    static __device__ __forceinline__ void inv2(float r2, float &r2inv, float &rinv) {
+        r2inv = __builtin_amdgcn_rcpf(r2);
+        rinv = __builtin_amdgcn_rsqf(r2);
+    }
 };
 
 template <typename R> struct PO {
@@ -45,8 +57,8 @@ template <typename R> struct PO {
 // _loopBodyCharge (VectorizedCellProcessor.cpp:227-277)
 template <typename R> __device__ __forceinline__ void body_cc(V3<R> d, R q1, R q2, PO<R> &o) {
     R r2 = dot3(d, d);
-    R r2inv = Math<R>::rcp(r2);
-    R rinv = Math<R>::rinv(r2, r2inv);
+    R r2inv, rinv;
+    Math<R>::inv2(r2, r2inv, rinv);
     R upot = q1 * q2 * rinv;
     R fac = upot * r2inv;
     o.f = {d.x * fac, d.y * fac, d.z * fac};
@@ -58,8 +70,8 @@ template <typename R> __device__ __forceinline__ void body_cc(V3<R> d, R q1, R q
 template <typename R>
 __device__ __forceinline__ void body_cd(V3<R> d, R q, V3<R> e, R p, PO<R> &o) {
     R r2 = dot3(d, d);
-    R r2inv = Math<R>::rcp(r2);
-    R rinv = Math<R>::rinv(r2, r2inv);
+    R r2inv, rinv;
+    Math<R>::inv2(r2, r2inv, rinv);
     R re = dot3(d, e);
     R qpr3 = q * p * (r2inv * rinv);
     R fac = r2inv * R(3) * re;
@@ -74,8 +86,8 @@ template <typename R>
 __device__ __forceinline__ void body_dd(V3<R> d, V3<R> e1, R p1, V3<R> e2, R p2, R epsRFInvrc3,
                                         PO<R> &o) {
     R r2 = dot3(d, d);
-    R r2inv = Math<R>::rcp(r2);
-    R rinv = Math<R>::rinv(r2, r2inv);
+    R r2inv, rinv;
+    Math<R>::inv2(r2, r2inv, rinv);
     R r2three = R(3) * r2inv;
     R p1p2 = p1 * p2;
     R rffac = p1p2 * epsRFInvrc3;
@@ -104,8 +116,8 @@ __device__ __forceinline__ void body_dd(V3<R> d, V3<R> e1, R p1, V3<R> e2, R p2,
 template <typename R>
 __device__ __forceinline__ void body_cq(V3<R> d, R q, V3<R> e, R m, PO<R> &o) {
     R r2 = dot3(d, d);
-    R r2inv = Math<R>::rcp(r2);
-    R rinv = Math<R>::rinv(r2, r2inv);
+    R r2inv, rinv;
+    Math<R>::inv2(r2, r2inv, rinv);
     R costj = dot3(e, d) * rinv;
     R qQr3 = R(0.5) * q * m * rinv * r2inv;
     R upot = qQr3 * (R(3) * costj * costj - R(1));
@@ -123,8 +135,8 @@ __device__ __forceinline__ void body_cq(V3<R> d, R q, V3<R> e, R m, PO<R> &o) {
 template <typename R>
 __device__ __forceinline__ void body_dq(V3<R> d, V3<R> ei, R p, V3<R> ej, R m, PO<R> &o) {
     R r2 = dot3(d, d);
-    R r2inv = Math<R>::rcp(r2);
-    R rinv = Math<R>::rinv(r2, r2inv);
+    R r2inv, rinv;
+    Math<R>::inv2(r2, r2inv, rinv);
     R myqfac = R(1.5) * p * m * r2inv * r2inv;
     R costi = dot3(ei, d) * rinv, costj = dot3(ej, d) * rinv;
     R cosgij = dot3(ei, ej);
@@ -148,8 +160,8 @@ __device__ __forceinline__ void body_dq(V3<R> d, V3<R> ei, R p, V3<R> ej, R m, P
 template <typename R>
 __device__ __forceinline__ void body_qq(V3<R> d, V3<R> ei, R mi, V3<R> ej, R mj, PO<R> &o) {
     R r2 = dot3(d, d);
-    R r2inv = Math<R>::rcp(r2);
-    R rinv = Math<R>::rinv(r2, r2inv);
+    R r2inv, rinv;
+    Math<R>::inv2(r2, r2inv, rinv);
     R qfac = R(0.75) * rinv * (mi * mj) * (r2inv * r2inv);
     R costi = dot3(ei, d) * rinv, costj = dot3(ej, d) * rinv;
     R c2i = costi * costi, c2j = costj * costj;
@@ -265,6 +277,7 @@ template <typename R> struct ISite {
     V3<R> pos, mol, e;
     R par;
     int ljid;
+    int ljrow;          // ljid * nlj * 4: the row of the interleaved LJ table
     int idx;            // site index (FP64 molecule position for the cutoff re-check)
 };
 
@@ -312,8 +325,9 @@ template <int TI, typename R, bool MOLSITE, bool MULTI> struct TileCfg {
 template <int TI, typename R, bool MOLSITE, bool MULTI> struct Tile {
     using C = TileCfg<TI, R, MOLSITE, MULTI>;
     Q4<R> p[C::CAP];                   // site position; w = charge / moment / LJ id
-    Q4<R> m[C::MOL ? C::CAP : 1];      // molecule position; w = its site count at
-                                       // the molecule's first site, 0 elsewhere
+    Q4<R> m[C::MOL ? C::CAP : 1];      // molecule position; w = |m|^2 (FP32 tiles)
+    // site count of a partner molecule at its first site (charge partners)
+    unsigned char ns[C::MOL && TI != T_LJ ? C::CAP : 1];
     // first-site (head) indices of a tile of charges (partners tested per molecule)
     unsigned short h[C::MOL && TI != T_LJ ? C::CAP : 1];
     Q4<R> e[C::ORI ? C::CAP : 1];      // orientation (dipole / quadrupole axis)

@@ -62,11 +62,20 @@ __device__ __forceinline__ void eval_lj(const PairArgs &a, const ISite<R> &I, co
     V3<R> dm = d;
     if constexpr (!MOLSITE) dm = {I.mol.x - mj.x, I.mol.y - mj.y, I.mol.z - mj.z};
     if constexpr (MULTI) {
-        const int k = I.ljid * a.pp.nlj + (int)pj.w;
+        // pj.w carries the partner's column offset (4 * its LJ id) as integer
+        // bits in the low word
         if constexpr (sizeof(R) == 8) {
-            eps24 = a.pp.eps24[k]; sig2 = a.pp.sig2[k]; shift6 = a.pp.shift6[k];
+            const double *t = a.pp.dpar + (unsigned)(I.ljrow + __double2loint(pj.w));
+            const double2 es = *reinterpret_cast<const double2 *>(t);
+            eps24 = es.x;
+            sig2 = es.y;
+            shift6 = t[2];
         } else {
-            eps24 = a.pp.feps24[k]; sig2 = a.pp.fsig2[k]; shift6 = a.pp.fshift6[k];
+            const float4 t =
+                *reinterpret_cast<const float4 *>(a.pp.fpar + (unsigned)(I.ljrow + __float_as_int(pj.w)));
+            eps24 = t.x;
+            sig2 = t.y;
+            shift6 = t.z;
         }
     }
     const R far = sizeof(R) == 4 ? R(__builtin_inff()) : R(1e300);
@@ -196,18 +205,26 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
     const SiteArrays &J = a.s[TJ];
     const double rc2 = (TI == T_LJ) ? a.rclj2 : a.rc2;
     const R rc2r = (R)rc2;
-    // FP32 LJ tiles: the test record's w holds |x|^2 and stage A computes
+    // FP32 tiles: the test record's w holds |x|^2 and stage A computes
     // r^2 = (|i|^2 + |j|^2) - 2 i.j (4 instead of 6 operations); its rounding
     // stays below 1.5e-5 (cl / rc)^2 rc^2 (coordinates within two cell edges
     // of the i cell's corner), so stage A keeps candidates up to that much
     // above rc^2 and stage B re-decides the band in FP64 as before
-    constexpr bool NORM = TI == T_LJ && sizeof(R) == 4 && (C::MOL || !C::LJID);
+    constexpr bool NORM = sizeof(R) == 4 && (C::MOL || (TI == T_LJ && !C::LJID));
     // idle lanes test from far away: r^2 overflows or exceeds any cutoff
     const V3<R> far = {R(1e30), R(1e30), R(1e30)};
     const V3<R> Imol = valid ? I.mol : far, Ipos = valid ? I.pos : far;
     // FP32 re-check band around rc^2 (relative 1e-5, far above FP32 rounding
     // of cell-relative coordinates)
     const R lo = rc2r * R(1 - 1e-5), hi = rc2r * R(1 + 1e-5);
+    // stage-A bound: FP64 rc^2; FP32 the top of the re-check band, widened
+    // by the rounding bound of the expanded r^2 where that is used
+    R cut = sizeof(R) == 4 ? hi : rc2r;
+    if constexpr (NORM) {
+        const double clm = fmax(g.cl[0], fmax(g.cl[1], g.cl[2]));
+        const double band = fmax(1e-5, 1.5e-5 * clm * clm / rc2);
+        cut = (R)(rc2 * (1.0 + band));
+    }
     unsigned long long nhit = 0;
     int nk = 0;
     if (lane < 27) {
@@ -271,7 +288,12 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             q.y = p.y + off.y;
             q.z = p.z + off.z;
             q.w = R(0);
-            if constexpr (C::LJID) q.w = (R)J.ljid[j];
+            if constexpr (C::LJID) {  // column offset of the interleaved table, as bits
+                if constexpr (sizeof(R) == 8)
+                    q.w = __hiloint2double(0, 4 * J.ljid[j]);
+                else
+                    q.w = __int_as_float(4 * J.ljid[j]);
+            }
             if constexpr (TI != T_LJ) q.w = SiteView<R>::par(J, j);
             if constexpr (NORM && !C::MOL) q.w = q.x * q.x + q.y * q.y + q.z * q.z;
             T.p[s] = q;
@@ -281,12 +303,15 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                     head = idx == T.pre[k];
                     if (!head) head = J.mol[j - 1] != J.mol[j];
                 }
-                R w = R(0);
-                if (head) {
-                    const int mo = J.mol[j];
-                    w = (R)(J.mstart[mo + 1] - J.mstart[mo]);
+                if constexpr (heads) {
+                    int w = 0;
+                    if (head) {
+                        const int mo = J.mol[j];
+                        w = J.mstart[mo + 1] - J.mstart[mo];
+                    }
+                    T.ns[s] = (unsigned char)w;
                 }
-                Q4<R> mq{m.x + off.x, m.y + off.y, m.z + off.z, w};
+                Q4<R> mq{m.x + off.x, m.y + off.y, m.z + off.z, R(0)};
                 if constexpr (NORM) mq.w = mq.x * mq.x + mq.y * mq.y + mq.z * mq.z;
                 T.m[s] = mq;
             }
@@ -383,9 +408,11 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                     sn = (int)T.lst[0][lane];
                     qn = C::MOL ? T.m[sn] : T.p[sn];
                 }
-                for (int e = 0;; ++e) {
+                int emax = cnt;  // the longest list: a scalar trip count
+                for (int o = 32; o > 0; o >>= 1) emax = max(emax, __shfl_xor(emax, o, 64));
+                emax = __builtin_amdgcn_readfirstlane(emax);
+                for (int e = 0; e < emax; ++e) {
                     const bool act = e < cnt;
-                    if (!__any(act)) break;
                     const int sj = sn;
                     const Q4<R> qj = qn;
                     if (e + 1 < cnt) {
@@ -394,7 +421,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                     }
                     if (act && accept(sj, qj)) {
                         if constexpr (heads) {
-                            const int ns = (int)qj.w;  // the molecule's sites sj .. sj + ns - 1
+                            const int ns = (int)T.ns[sj];  // the molecule's sites sj .. sj + ns - 1
                             for (int t = 0; t < ns; ++t) {
                                 const Q4<R> pj = T.p[sj + t];
                                 eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj + t, pj, qj,
@@ -418,8 +445,10 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         // full are inside [0, nst) for every phase and need no bounds test,
         // and idle lanes carry a far-away position, so the bulk loop is a
         // read, a distance and a compare per candidate.
-        // wave-uniform trip counts (scalar loop control)
-        const int per = __builtin_amdgcn_readfirstlane((nh + P - 1) / P), full = per - 1;
+        // wave-uniform trip counts (scalar loop control; nh and P are
+        // wave-uniform, so the division runs on the scalar unit)
+        const int nhs = __builtin_amdgcn_readfirstlane(nh), Ps = __builtin_amdgcn_readfirstlane(P);
+        const int per = (nhs + Ps - 1) / Ps, full = per - 1;
         const Q4<R> *rec = (C::MOL ? T.m : T.p) + ph;
         // candidate c of the stage-A sequence -> tile index
         auto cand = [&](int c) -> int {
@@ -427,13 +456,9 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             else return c;
         };
         const V3<R> mi = C::MOL ? Imol : Ipos;
-        R cut = sizeof(R) == 4 ? hi : rc2r;
         // NORM: -2 i and |i|^2 per lane (idle lanes: far away, never a hit)
         R mi2 = R(0), nx = R(0), ny = R(0), nz = R(0);
         if constexpr (NORM) {
-            const double clm = fmax(g.cl[0], fmax(g.cl[1], g.cl[2]));
-            const double band = fmax(1e-5, 1.5e-5 * clm * clm / rc2);
-            cut = (R)(rc2 * (1.0 + band));
             mi2 = mi.x * mi.x + mi.y * mi.y + mi.z * mi.z;
             nx = R(-2) * mi.x;
             ny = R(-2) * mi.y;
@@ -545,9 +570,10 @@ __device__ __forceinline__ bool pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             I.mol = MOLSITE ? I.pos : SiteView<R>::mol(S, i);
             I.idx = i;
             if constexpr (TI >= T_D) I.e = SiteView<R>::ori(S, i);
-            if constexpr (TI == T_LJ)
+            if constexpr (TI == T_LJ) {
                 I.ljid = S.ljid[i];
-            else
+                I.ljrow = 4 * a.pp.nlj * I.ljid;
+            } else
                 I.par = SiteView<R>::par(S, i);
             Acc acc;
             for (int k = 0; k < 3; ++k) acc.f[k] = acc.v[k] = acc.m[k] = 0.0;
