@@ -66,7 +66,7 @@ def measured_traffic(workload, scale, tag, shape, kernel):
     summary covers this workload, form, launch shape and kernel."""
     if scale != 1.0:
         return None
-    f = ROOT / "profiles" / f"traffic_{workload}.json"
+    f = ROOT / "profiles" / f"k_pairs_{workload}_traffic.json"
     try:
         d = json.loads(f.read_text())
         return d[tag]["%d,%d" % tuple(shape)][kernel]["traffic_bytes"]
