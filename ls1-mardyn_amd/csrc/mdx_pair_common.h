@@ -324,16 +324,42 @@ template <int TI, typename R, bool MOLSITE, bool MULTI> struct TileCfg {
 
 template <int TI, typename R, bool MOLSITE, bool MULTI> struct Tile {
     using C = TileCfg<TI, R, MOLSITE, MULTI>;
-    Q4<R> p[C::CAP];                   // site position; w = charge / moment / LJ id
-    Q4<R> m[C::MOL ? C::CAP : 1];      // molecule position; w = |m|^2 (FP32 tiles)
-    // site count of a partner molecule at its first site (charge partners)
-    unsigned char ns[C::MOL && TI != T_LJ ? C::CAP : 1];
-    // first-site (head) indices of a tile of charges (partners tested per molecule)
-    unsigned short h[C::MOL && TI != T_LJ ? C::CAP : 1];
-    Q4<R> e[C::ORI ? C::CAP : 1];      // orientation (dipole / quadrupole axis)
+    // CAP slots of the largest record set (site, molecule, orientation,
+    // head index and site count); each partner pass lays out only the records
+    // its partner type needs (PassTile), so a pass with smaller records holds
+    // more sites in the same bytes
+    static constexpr int SLOT =
+        (int)sizeof(Q4<R>) * (1 + C::MOL + C::ORI) + (C::MOL && TI != T_LJ ? 3 : 0);
+    alignas(32) unsigned char raw[C::CAP * SLOT];
     unsigned short lst[C::LIST][64];  // per-lane hit lists (tile indices)
     int j0[27], pre[28];
     double shift[27][3];
+};
+
+// The records of one partner pass (partner type TJ) in a tile's slot bytes:
+// p site position (w = charge / moment, LJ table column, or |x|^2 in FP32
+// one-centre tiles), m molecule position (w = |m|^2 in FP32 tiles), e
+// orientation (dipole and quadrupole partners), h first-site (head) indices
+// and ns site counts at heads (charge partners, tested per molecule).
+template <int TI, int TJ, typename R, bool MOLSITE, bool MULTI> struct PassTile {
+    using C = TileCfg<TI, R, MOLSITE, MULTI>;
+    using TT = Tile<TI, R, MOLSITE, MULTI>;
+    static constexpr bool HEADS = C::MOL && TJ == T_C && TI != T_LJ;
+    static constexpr bool ORIJ = TI != T_LJ && TJ >= T_D;
+    static constexpr int REC = 1 + C::MOL + ORIJ;
+    static constexpr int CAP =
+        (C::CAP * TT::SLOT) / ((int)sizeof(Q4<R>) * REC + (HEADS ? 3 : 0));
+    static_assert(CAP >= C::CAP && CAP < 65536, "pass capacity");
+    Q4<R> *p, *m, *e;
+    unsigned short *h;
+    unsigned char *ns;
+    __device__ __forceinline__ explicit PassTile(TT &t) {
+        p = reinterpret_cast<Q4<R> *>(t.raw);
+        m = p + (C::MOL ? CAP : 0);
+        e = p + CAP * (1 + C::MOL);
+        h = reinterpret_cast<unsigned short *>(p + CAP * REC);
+        ns = reinterpret_cast<unsigned char *>(h + (HEADS ? CAP : 0));
+    }
 };
 
 template <typename R>

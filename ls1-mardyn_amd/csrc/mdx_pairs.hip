@@ -104,15 +104,15 @@ __device__ __forceinline__ void eval_lj(const PairArgs &a, const ISite<R> &I, co
     }
 }
 
-// One electrostatic site pair seen from site i (tile entry s): force on i, virial
+// One electrostatic site pair seen from site i: force on i, virial
 // (m_i - m_j) * f_i, torque on i; roles swapped where the reference swaps.
-// pj = T.p[s]; mj = T.m[s] where the tile keeps molecule positions (read
-// ahead by the caller).
+// pj, mj, ej: the partner's site, molecule and orientation records (the
+// caller reads them from its pass's tile records; ej only for dipole and
+// quadrupole partners).
 template <int TI, int TJ, typename R, bool MOLSITE, bool MULTI>
 __device__ __forceinline__ void eval_pair(const PairArgs &a, const ISite<R> &I,
-                                          const Tile<TI, R, MOLSITE, MULTI> &T, int s,
-                                          const Q4<R> &pj, const Q4<R> &mj, R eps24, R sig2,
-                                          R shift6, Acc &acc, Part32 &pa) {
+                                          const Q4<R> &eq, const Q4<R> &pj, const Q4<R> &mj,
+                                          R eps24, R sig2, R shift6, Acc &acc, Part32 &pa) {
     V3<R> d = {I.pos.x - pj.x, I.pos.y - pj.y, I.pos.z - pj.z};
     V3<R> dm = d;
     if constexpr (!MOLSITE) dm = {I.mol.x - mj.x, I.mol.y - mj.y, I.mol.z - mj.z};
@@ -122,7 +122,7 @@ __device__ __forceinline__ void eval_pair(const PairArgs &a, const ISite<R> &I,
     V3<R> tq = {R(0), R(0), R(0)};
     static_assert(TI != T_LJ, "LJ pairs go through eval_lj");
     V3<R> ej = {R(0), R(0), R(0)};
-    if constexpr (TJ >= T_D) ej = v3(T.e[s]);
+    if constexpr (TJ >= T_D) ej = v3(eq);
     const R qj = pj.w;
     if constexpr (TI == T_C && TJ == T_C) {
         body_cc<R>(d, I.par, qj, o);
@@ -256,6 +256,9 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
     // the indirection costs more than it saves (measured) and every site is a
     // candidate.
     constexpr bool heads = C::MOL && TJ == T_C;
+    using PT = PassTile<TI, TJ, R, MOLSITE, MULTI>;
+    static_assert(PT::HEADS == heads, "head layout");
+    const PT V(T);
     auto is_head = [&](int idx) -> bool {
         const int k = tile_k(T.pre, idx);
         if (idx == T.pre[k]) return true;  // a cell range starts with a molecule
@@ -263,7 +266,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         return J.mol[j - 1] != J.mol[j];
     };
     for (int base = 0; base < total;) {
-        int nst = min(C::CAP, total - base);
+        int nst = min(PT::CAP, total - base);
         if constexpr (heads) {
             if (base + nst < total) {  // back off to the first site of the split molecule
                 const int t = lane;
@@ -296,7 +299,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             }
             if constexpr (TI != T_LJ) q.w = SiteView<R>::par(J, j);
             if constexpr (NORM && !C::MOL) q.w = q.x * q.x + q.y * q.y + q.z * q.z;
-            T.p[s] = q;
+            V.p[s] = q;
             if constexpr (C::MOL) {
                 const V3<R> m = SiteView<R>::mol(J, j);
                 if constexpr (heads) {
@@ -309,20 +312,20 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                         const int mo = J.mol[j];
                         w = J.mstart[mo + 1] - J.mstart[mo];
                     }
-                    T.ns[s] = (unsigned char)w;
+                    V.ns[s] = (unsigned char)w;
                 }
                 Q4<R> mq{m.x + off.x, m.y + off.y, m.z + off.z, R(0)};
                 if constexpr (NORM) mq.w = mq.x * mq.x + mq.y * mq.y + mq.z * mq.z;
-                T.m[s] = mq;
+                V.m[s] = mq;
             }
             if constexpr (TI != T_LJ && TJ >= T_D) {
                 const V3<R> e = SiteView<R>::ori(J, j);
-                T.e[s] = Q4<R>{e.x, e.y, e.z, R(0)};
+                V.e[s] = Q4<R>{e.x, e.y, e.z, R(0)};
             }
             }
             if constexpr (heads) {  // compact the heads in tile order
                 const unsigned long long hm = __ballot(head);
-                if (head) T.h[hb + __popcll(hm & ((1ull << lane) - 1ull))] = (unsigned short)s;
+                if (head) V.h[hb + __popcll(hm & ((1ull << lane) - 1ull))] = (unsigned short)s;
                 hb += __popcll(hm);
             }
         }
@@ -372,7 +375,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                     const bool act = e < cnt;
                     const int raw = (int)T.lst[e][lane];  // e < LIST: in bounds
                     const int sj = act ? raw : 0;
-                    const Q4<R> qj = C::MOL ? T.m[sj] : T.p[sj];
+                    const Q4<R> qj = C::MOL ? V.m[sj] : V.p[sj];
                     const V3<R> mi = C::MOL ? I.mol : I.pos;
                     const R dx = mi.x - qj.x, dy = mi.y - qj.y, dz = mi.z - qj.z;
                     const R r2 = dx * dx + dy * dy + dz * dz;
@@ -382,7 +385,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                         band |= b ? (1u << e) : 0u;
                         ok = ok && !b;
                     }
-                    const Q4<R> pj = C::MOL ? T.p[sj] : qj;
+                    const Q4<R> pj = C::MOL ? V.p[sj] : qj;
                     eval_lj<R, MOLSITE, MULTI>(a, I, pj, qj, ok, eps24, sig2, shift6, acc, pa);
                     if (COUNT) nhit += ok;
                 }
@@ -393,8 +396,8 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                         band &= has ? band - 1u : 0u;
                         const int sj = has ? (int)T.lst[e][lane] : 0;
                         const bool ok = has && recheck(sj);
-                        const Q4<R> qj = C::MOL ? T.m[sj] : T.p[sj];
-                        const Q4<R> pj = C::MOL ? T.p[sj] : qj;
+                        const Q4<R> qj = C::MOL ? V.m[sj] : V.p[sj];
+                        const Q4<R> pj = C::MOL ? V.p[sj] : qj;
                         eval_lj<R, MOLSITE, MULTI>(a, I, pj, qj, ok, eps24, sig2, shift6, acc, pa);
                         if (COUNT) nhit += ok;
                     }
@@ -406,7 +409,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                 Q4<R> qn{};
                 if (cnt > 0) {
                     sn = (int)T.lst[0][lane];
-                    qn = C::MOL ? T.m[sn] : T.p[sn];
+                    qn = C::MOL ? V.m[sn] : V.p[sn];
                 }
                 int emax = cnt;  // the longest list: a scalar trip count
                 for (int o = 32; o > 0; o >>= 1) emax = max(emax, __shfl_xor(emax, o, 64));
@@ -417,21 +420,23 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                     const Q4<R> qj = qn;
                     if (e + 1 < cnt) {
                         sn = (int)T.lst[e + 1][lane];
-                        qn = C::MOL ? T.m[sn] : T.p[sn];
+                        qn = C::MOL ? V.m[sn] : V.p[sn];
                     }
                     if (act && accept(sj, qj)) {
                         if constexpr (heads) {
-                            const int ns = (int)T.ns[sj];  // the molecule's sites sj .. sj + ns - 1
+                            const int ns = (int)V.ns[sj];  // the molecule's sites sj .. sj + ns - 1
                             for (int t = 0; t < ns; ++t) {
-                                const Q4<R> pj = T.p[sj + t];
-                                eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj + t, pj, qj,
+                                const Q4<R> pj = V.p[sj + t];
+                                eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, Q4<R>{}, pj, qj,
                                                                      eps24, sig2, shift6, acc, pa);
                             }
                             if (COUNT) nhit += ns;
                         } else {
-                            const Q4<R> pj = C::MOL ? T.p[sj] : qj;
-                            eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, T, sj, pj, qj, eps24,
-                                                                 sig2, shift6, acc, pa);
+                            const Q4<R> pj = C::MOL ? V.p[sj] : qj;
+                            Q4<R> eq{};
+                            if constexpr (PT::ORIJ) eq = V.e[sj];
+                            eval_pair<TI, TJ, R, MOLSITE, MULTI>(a, I, eq, pj, qj, eps24, sig2,
+                                                                 shift6, acc, pa);
                             if (COUNT) ++nhit;
                         }
                     }
@@ -449,10 +454,10 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         // wave-uniform, so the division runs on the scalar unit)
         const int nhs = __builtin_amdgcn_readfirstlane(nh), Ps = __builtin_amdgcn_readfirstlane(P);
         const int per = (nhs + Ps - 1) / Ps, full = per - 1;
-        const Q4<R> *rec = (C::MOL ? T.m : T.p) + ph;
+        const Q4<R> *rec = (C::MOL ? V.m : V.p) + ph;
         // candidate c of the stage-A sequence -> tile index
         auto cand = [&](int c) -> int {
-            if constexpr (heads) return (int)T.h[c];
+            if constexpr (heads) return (int)V.h[c];
             else return c;
         };
         const V3<R> mi = C::MOL ? Imol : Ipos;
@@ -484,7 +489,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                 Q4<R> q;
                 if constexpr (heads) {
                     sv[u] = cand(ph + (m0 + u) * P);
-                    q = T.m[sv[u]];
+                    q = V.m[sv[u]];
                 } else {
                     sv[u] = ph + (m0 + u) * P;
                     q = rec[(m0 + u) * P];
@@ -505,7 +510,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             if (__any(cnt > C::LIST - 1)) drain();
             const int c = ph + m0 * P;
             const int sidx = cand(min(c, nh - 1));
-            const Q4<R> q = C::MOL ? T.m[sidx] : T.p[sidx];
+            const Q4<R> q = C::MOL ? V.m[sidx] : V.p[sidx];
             const R r2 = dist2(q);
             const bool in = valid && c < nh;
             if (COUNT && in) ++ntest;
