@@ -7,6 +7,7 @@ namespace mdx {
 
 constexpr int kAdjMaxLJ = 16;   // LJ lookup ids supported by the adjoint
 constexpr int kAdjBlocks = 4096; // upper bound on adjoint blocks (partials rows)
+constexpr int kAdjPar = 6;       // per-pair constants of the LJ adjoint (LDS)
 
 struct AdjArgs {
     Geom g;

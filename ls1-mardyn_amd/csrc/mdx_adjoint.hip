@@ -43,11 +43,13 @@ template <int TI>
 __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
     __shared__ AdjTile T;
     extern __shared__ double lds[];
-    // per-lane columns [W][64], the block table [K*W], lane keys [64]
+    // per-lane columns [W][64], the block table [K*W], lane keys [64], then
+    // (LJ) the per-pair constants [nl * nl][kAdjPar]
     const int W = a.W, K = a.K, TW = K * W;
     double *col = lds;
     double *table = lds + (size_t)W * 64;
     int *lkey = (int *)(table + TW);
+    double *ptab = (double *)(lkey + 64);
     const int lane = threadIdx.x;
     for (int e = lane; e < TW; e += 64) table[e] = 0.0;
     for (int w = 0; w < W; ++w) col[w * 64 + lane] = 0.0;
@@ -55,7 +57,42 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
     const double rc2 = TI == T_LJ ? a.g.rclj2 : a.g.rc2;
     const int nl = a.nlj;
     const bool vjp = a.cx != nullptr;
-    for (int cell = blockIdx.x; cell < a.ncell; cell += gridDim.x) {
+    if constexpr (TI == T_LJ) {
+        // the parameter-only factors of the derivative terms, once per block:
+        // 4 eps / sigma, 24 eps / sigma and the truncation-shift terms of
+        // dU/deps and dU/dsigma (zero where no shift is tied to the pair)
+        for (int kk = lane; kk < nl * nl; kk += 64) {
+            const double e24 = a.pp.eps24[kk], s2 = a.pp.sig2[kk];
+            const double eps = e24 / 24.0, sig = sqrt(s2);
+            double cue = 0.0, cus = 0.0;
+            const double rcs = a.shift_rc[kk];
+            if (rcs > 0.0) {
+                double s6 = s2 / (rcs * rcs);
+                s6 = s6 * s6 * s6;
+                cue = 4.0 * (s6 - s6 * s6);
+                cus = 4.0 * eps * (6.0 * s6 - 12.0 * s6 * s6) / sig;
+            }
+            double *pt = ptab + (size_t)kk * kAdjPar;
+            pt[0] = e24;
+            pt[1] = s2;
+            pt[2] = 4.0 * eps / sig;
+            pt[3] = e24 / sig;
+            pt[4] = cue;
+            pt[5] = cus;
+        }
+        __syncthreads();
+    }
+    // cells: XCD-aware where the grid allows it (block b runs on XCD b % 8,
+    // which walks a contiguous eighth of the cells, so neighbour sites are
+    // shared through that XCD's L2), else a plain stride
+    int cfirst = blockIdx.x, cend = a.ncell, cstride = gridDim.x;
+    if (gridDim.x >= 8 && (gridDim.x & 7) == 0) {
+        const int per = (a.ncell + 7) / 8, x = blockIdx.x & 7;
+        cfirst = x * per + (blockIdx.x >> 3);
+        cend = min(a.ncell, (x + 1) * per);
+        cstride = gridDim.x >> 3;
+    }
+    for (int cell = cfirst; cell < cend; cell += cstride) {
         const int i_begin = S.cstart[cell], i_end = S.cstart[cell + 1];
         if (i_begin == i_end) continue;
         const int cx = cell % a.g.n[0], cy = (cell / a.g.n[0]) % a.g.n[1],
@@ -131,7 +168,8 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                         const Q4<double> pj = T.p[t], mj4 = T.m[t];
                         const double dmx = mx - mj4.x, dmy = my - mj4.y, dmz = mz - mj4.z;
                         const double dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
-                        const double r2 = dx * dx + dy * dy + dz * dz, r2i = 1.0 / r2;
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        const double r2i = Math<double>::rcp(r2);
                         const double dmd = dmx * dx + dmy * dy + dmz * dz;
                         double dc[3] = {0, 0, 0};
                         if (vjp) {
@@ -142,23 +180,17 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                         double sc, sp;
                         if constexpr (TI == T_LJ) {
                             const int bj = (int)pj.w, kk = key * nl + bj;
-                            const double e24 = a.pp.eps24[kk], s2 = a.pp.sig2[kk];
-                            const double eps = e24 / 24.0, sig = sqrt(s2);
+                            const double *pt = ptab + (size_t)kk * kAdjPar;
+                            const double e24 = pt[0], s2 = pt[1];
                             double l6 = s2 * r2i;
                             l6 = l6 * l6 * l6;
                             const double l12 = l6 * l6;
                             sc = e24 * r2i * (2.0 * l12 - l6);
                             sp = e24 * r2i * r2i * (4.0 * l6 - 14.0 * l12);
-                            double gUe = 4.0 * (l12 - l6), gUs = 4.0 * eps * (12.0 * l12 - 6.0 * l6) / sig;
-                            const double rcs = a.shift_rc[kk];
-                            if (rcs > 0.0) {
-                                double s6 = s2 / (rcs * rcs);
-                                s6 = s6 * s6 * s6;
-                                gUe += 4.0 * (s6 - s6 * s6);
-                                gUs += 4.0 * eps * (6.0 * s6 - 12.0 * s6 * s6) / sig;
-                            }
+                            const double gUe = 4.0 * (l12 - l6) + pt[4];
+                            const double gUs = pt[2] * (12.0 * l12 - 6.0 * l6) + pt[5];
                             const double gWe = dmd * 24.0 * r2i * (2.0 * l12 - l6);
-                            const double gWs = dmd * e24 * r2i * (24.0 * l12 - 6.0 * l6) / sig;
+                            const double gWs = dmd * pt[3] * r2i * (24.0 * l12 - 6.0 * l6);
                             double *c0 = col + (size_t)(bj * 4) * 64 + lane;
                             c0[0] += 0.5 * gUe;
                             c0[64] += 0.5 * gUs;
@@ -166,7 +198,7 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                             c0[192] += 0.5 * gWs;
                         } else {
                             const double qj = pj.w;
-                            const double ri = sqrt(r2i), r3i = r2i * ri;
+                            const double ri = Math<double>::rinv(r2, r2i), r3i = r2i * ri;
                             col[lane] += qj * ri;
                             col[64 + lane] += qj * dmd * r3i;
                             sc = qi * qj * r3i;

@@ -1911,7 +1911,11 @@ int adjoint_check(mdx_context *c, int64_t n, const double *cot, const double *vj
     return MDX_OK;
 }
 
-int adjoint_grid(const mdx_context *c) { return std::max(1, std::min(c->ncell, kAdjBlocks)); }
+// a multiple of 8 from 8 cells up (the kernel's XCD-aware cell ranges)
+int adjoint_grid(const mdx_context *c) {
+    const int g = std::max(1, std::min(c->ncell, kAdjBlocks));
+    return g >= 8 ? g & ~7 : g;
+}
 
 // Largest per-block table (keys x values) of the LJ and charge passes.
 size_t adjoint_table_width(const mdx_context *c) {
@@ -1968,7 +1972,8 @@ int adjoint_run(mdx_context *c, const double *cot, double *dU_deps, double *dU_d
         a.ncell = c->ncell;
         a.K = t == T_LJ ? nl : ncomp * kMaxSite;
         a.W = t == T_LJ ? 4 * nl : 2;
-        const size_t lds = ((size_t)a.W * 64 + (size_t)a.K * a.W) * 8 + 64 * 4;
+        const size_t lds = ((size_t)a.W * 64 + (size_t)a.K * a.W) * 8 + 64 * 4 +
+                           (t == T_LJ ? (size_t)nl * nl * kAdjPar * 8 : 0);
         hipLaunchKernelGGL(adjoint_kernel(t), dim3(grid), dim3(64), lds, c->stream, a);
         const int width = a.K * a.W;
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->stream, b.part, grid,

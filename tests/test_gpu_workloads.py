@@ -119,6 +119,20 @@ def test_mixture_rc3():
     check_trajectory(gpu_eval(s, PREC_FP64, steps=steps), traj, s.box)
 
 
+def test_mixture_dense_cells():
+    """The mixture on 3 x 3 x 3 cells (rc 4.9, about 59 molecules per cell):
+    each cell's charges span two 64-lane chunks, and every charge and
+    quadrupole pass walks the whole box in many LDS chunks, split at molecule
+    boundaries for charge partners."""
+    s = scenarios.mixture(n=1600, rc=4.9)
+    say("case mixture dense cells N", s.n, "rc", s.rc, "box", s.box[0])
+    assert int(s.box[0] / s.rc) == 3
+    o = oracle_for(s)
+    ref = o.forces(s.r, s.q, s.cid)
+    check_fp64(gpu_eval(s, PREC_FP64), ref)
+    check_fp32(gpu_eval(s, PREC_FP32_FP64ACC), ref)
+
+
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32_FP64ACC])
 def test_replica_batch_each_vs_oracle(prec):
     reps = scenarios.replicas(nrep=12, nmin=2000, nmax=4000)
