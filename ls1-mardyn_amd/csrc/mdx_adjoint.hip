@@ -29,6 +29,7 @@ namespace mdx {
 
 constexpr int kAdjCap = 128;  // partner sites per LDS chunk
 constexpr int kAdjList = 16;  // per-lane hit list entries
+constexpr int kAdjUnroll = 4; // stage-A rows per drain check
 
 struct AdjTile {
     Q4<double> p[kAdjCap];  // site position, i cell frame; w = LJ id or charge
@@ -162,7 +163,10 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                 int cnt = 0;
                 // stage B: the derivative terms of the listed partners
                 auto drain = [&]() {
-                    for (int e = 0; __any(e < cnt); ++e) {
+                    int emax = cnt;  // the longest list: a scalar trip count
+                    for (int o = 32; o > 0; o >>= 1) emax = max(emax, __shfl_xor(emax, o, 64));
+                    emax = __builtin_amdgcn_readfirstlane(emax);
+                    for (int e = 0; e < emax; ++e) {
                         if (e >= cnt) continue;
                         const int t = T.lst[e][lane];
                         const Q4<double> pj = T.p[t], mj4 = T.m[t];
@@ -211,18 +215,39 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                     cnt = 0;
                 };
                 // stage A: molecule-centre cutoff, strictly less; the molecule
-                // itself (distance 0) is skipped (SingleCellPolicy_)
-                for (int m = ph; m - ph < nst; m += P) {
-                    if (__any(cnt >= kAdjList)) drain();
-                    bool hit = false;
-                    if (m < nst) {
-                        const Q4<double> q = T.m[m];
-                        const double ex = mx - q.x, ey = my - q.y, ez = mz - q.z;
-                        const double r2m = ex * ex + ey * ey + ez * ez;
-                        hit = r2m < rc2 && r2m != 0.0;
+                // itself (distance 0) is skipped (SingleCellPolicy_). Row r
+                // holds candidates ph + r P; rows below full lie inside
+                // [0, nst) for every phase, so they run without bounds tests,
+                // kAdjUnroll rows per drain check; every candidate is written
+                // at the end of the list and kept only on a hit
+                auto hit = [&](int m) -> bool {
+                    const Q4<double> q = T.m[m];
+                    const double ex = mx - q.x, ey = my - q.y, ez = mz - q.z;
+                    const double r2m = ex * ex + ey * ey + ez * ez;
+                    return r2m < rc2 && r2m != 0.0;
+                };
+                const int nsts = __builtin_amdgcn_readfirstlane(nst);
+                const int Ps = __builtin_amdgcn_readfirstlane(P);
+                const int per = (nsts + Ps - 1) / Ps, full = per - 1;
+                int row = 0;
+                for (; row + kAdjUnroll <= full; row += kAdjUnroll) {
+                    if (__any(cnt > kAdjList - kAdjUnroll)) drain();
+                    bool h[kAdjUnroll];
+#pragma unroll
+                    for (int u = 0; u < kAdjUnroll; ++u) h[u] = hit(ph + (row + u) * P);
+#pragma unroll
+                    for (int u = 0; u < kAdjUnroll; ++u) {
+                        T.lst[cnt][lane] = (unsigned short)(ph + (row + u) * P);
+                        cnt += h[u] ? 1 : 0;
                     }
-                    T.lst[cnt][lane] = (unsigned short)m;
-                    cnt += hit ? 1 : 0;
+                }
+                for (; row < per; ++row) {  // the last rows, bounds-tested
+                    if (__any(cnt > kAdjList - 1)) drain();
+                    const int m = ph + row * P;
+                    const int mc = min(m, nst - 1);
+                    const bool h = hit(mc) && m < nst;
+                    T.lst[cnt][lane] = (unsigned short)mc;
+                    cnt += h ? 1 : 0;
                 }
                 drain();
                 __syncthreads();
