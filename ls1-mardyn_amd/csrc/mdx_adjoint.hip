@@ -40,20 +40,29 @@ struct AdjTile {
     double shift[27][3];
 };
 
-template <int TI>
+// NLR > 0: the LJ parameter sums of a lane (partner LJ ids < NLR) stay in
+// registers until the chunk's fold instead of read-modify-writes of its LDS
+// column per pair (charge sites always keep their two sums in registers)
+template <int TI, int NLR>
 __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
     __shared__ AdjTile T;
     extern __shared__ double lds[];
     // per-lane columns [W][64], the block table [K*W], lane keys [64], then
-    // (LJ) the per-pair constants [nl * nl][kAdjPar]
+    // (LJ) the per-pair constants [nl * nl][kAdjPar]. Kernels that keep the
+    // sums in registers fill the columns only for the fold after a chunk's
+    // partner passes, when the tile is free: their columns use its bytes.
+    constexpr bool ALIAS = TI != T_LJ || NLR > 0;
+    static_assert(!ALIAS || sizeof(AdjTile) >= (TI == T_LJ ? 4 * NLR : 2) * 64 * sizeof(double),
+                  "columns in the tile");
     const int W = a.W, K = a.K, TW = K * W;
-    double *col = lds;
-    double *table = lds + (size_t)W * 64;
+    double *col = ALIAS ? reinterpret_cast<double *>(&T) : lds;
+    double *table = ALIAS ? lds : lds + (size_t)W * 64;
     int *lkey = (int *)(table + TW);
     double *ptab = (double *)(lkey + 64);
     const int lane = threadIdx.x;
     for (int e = lane; e < TW; e += 64) table[e] = 0.0;
-    for (int w = 0; w < W; ++w) col[w * 64 + lane] = 0.0;
+    if constexpr (!ALIAS)
+        for (int w = 0; w < W; ++w) col[w * 64 + lane] = 0.0;
     const SiteArrays &S = a.s;
     const double rc2 = TI == T_LJ ? a.g.rclj2 : a.g.rc2;
     const int nl = a.nlj;
@@ -144,6 +153,9 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                 qi = S.p[i];
             }
             double g[3] = {0, 0, 0};
+            constexpr int NR = TI == T_LJ ? (NLR > 0 ? 4 * NLR : 1) : 2;
+            double racc[NR];
+            for (int w = 0; w < NR; ++w) racc[w] = 0.0;
             for (int base = 0; base < total; base += kAdjCap) {
                 const int nst = min(kAdjCap, total - base);
                 for (int t = lane; t < nst; t += 64) {
@@ -195,16 +207,27 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                             const double gUs = pt[2] * (12.0 * l12 - 6.0 * l6) + pt[5];
                             const double gWe = dmd * 24.0 * r2i * (2.0 * l12 - l6);
                             const double gWs = dmd * pt[3] * r2i * (24.0 * l12 - 6.0 * l6);
-                            double *c0 = col + (size_t)(bj * 4) * 64 + lane;
-                            c0[0] += 0.5 * gUe;
-                            c0[64] += 0.5 * gUs;
-                            c0[128] += 0.5 * gWe;
-                            c0[192] += 0.5 * gWs;
+                            if constexpr (NLR > 0) {
+#pragma unroll
+                                for (int b = 0; b < NLR; ++b) {
+                                    const bool mine = bj == b;
+                                    racc[4 * b] += mine ? 0.5 * gUe : 0.0;
+                                    racc[4 * b + 1] += mine ? 0.5 * gUs : 0.0;
+                                    racc[4 * b + 2] += mine ? 0.5 * gWe : 0.0;
+                                    racc[4 * b + 3] += mine ? 0.5 * gWs : 0.0;
+                                }
+                            } else {
+                                double *c0 = col + (size_t)(bj * 4) * 64 + lane;
+                                c0[0] += 0.5 * gUe;
+                                c0[64] += 0.5 * gUs;
+                                c0[128] += 0.5 * gWe;
+                                c0[192] += 0.5 * gWs;
+                            }
                         } else {
                             const double qj = pj.w;
                             const double ri = Math<double>::rinv(r2, r2i), r3i = r2i * ri;
-                            col[lane] += qj * ri;
-                            col[64 + lane] += qj * dmd * r3i;
+                            racc[0] += qj * ri;
+                            racc[1] += qj * dmd * r3i;
                             sc = qi * qj * r3i;
                             sp = -1.5 * qi * qj * r3i * r2i;
                         }
@@ -266,6 +289,12 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                 a.gy[i] = g[1];
                 a.gz[i] = g[2];
             }
+            // the register sums into the lane's columns (W <= NR of them)
+            if constexpr (TI != T_LJ || NLR > 0) {
+#pragma unroll
+                for (int w = 0; w < NR; ++w)
+                    if (w < W) col[w * 64 + lane] = racc[w];
+            }
             lkey[lane] = valid ? key : -1;
             __syncthreads();
             // fixed-order fold of the lane columns into the block table
@@ -277,7 +306,8 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                 table[e] += acc;
             }
             __syncthreads();
-            for (int w = 0; w < W; ++w) col[w * 64 + lane] = 0.0;
+            if constexpr (!ALIAS)
+                for (int w = 0; w < W; ++w) col[w * 64 + lane] = 0.0;
         }
     }
     for (int e = lane; e < TW; e += 64) a.part[(size_t)blockIdx.x * TW + e] = table[e];
@@ -311,7 +341,12 @@ __global__ void k_gather_orig3(int n, const int *__restrict__ orig, const double
     out[2 * n + i] = in[3 * o + 2];
 }
 
-AdjKernel adjoint_kernel(int ti) { return ti == T_LJ ? k_adjoint<T_LJ> : k_adjoint<T_C>; }
+AdjKernel adjoint_kernel(int ti, int nlj) {
+    if (ti != T_LJ) return k_adjoint<T_C, 0>;
+    if (nlj <= 2) return k_adjoint<T_LJ, 2>;
+    if (nlj <= 4) return k_adjoint<T_LJ, 4>;
+    return k_adjoint<T_LJ, 0>;
+}
 AdjMolKernel adjoint_mol_kernel() { return k_adjoint_mol; }
 GatherKernel gather_orig3_kernel() { return k_gather_orig3; }
 

@@ -49,6 +49,39 @@ def test_adjoint_matches_oracle(make):
         assert np.array_equal(got[k], again[k]), k
 
 
+def small_multi(ncomp, seed=83, n=360):
+    """One-centre LJ fluid of ncomp components (ncomp LJ lookup ids)."""
+    rho = 0.7
+    box = np.full(3, (n / rho) ** (1 / 3))
+    r = scenarios.rsa_positions(seed, [((0, 0, 0), tuple(box), n)], box, 0.85)
+    parts = []
+    for k in range(ncomp):
+        parts += scenarios.comp_1clj(1.0 - 0.07 * k, 0.95 + 0.03 * k, 1.0, 2.5, k % 2 == 1)
+    npair = ncomp * (ncomp - 1) // 2
+    mix = np.array([[1.0 + 0.01 * (k % 3), 1.0 - 0.02 * (k % 2)] for k in range(npair)])
+    rng = np.random.default_rng(seed)
+    cid = rng.integers(0, ncomp, n).astype(np.int32)
+    q = np.tile([1.0, 0, 0, 0], (n, 1))
+    return scenarios.System("multi", ncomp, np.array(parts), mix, 1.0, 2.5, 2.5, box, r,
+                            np.zeros((n, 3)), q, np.zeros((n, 3)), cid, 0.9)
+
+
+@pytest.mark.parametrize("ncomp", [3, 6])
+def test_adjoint_many_lj_ids(ncomp):
+    """3 LJ ids (parameter sums in registers, 4-id kernel) and 6 (sums in
+    per-lane LDS columns) against the oracle's adjoint."""
+    s = small_multi(ncomp)
+    print("case multi-component LJ", ncomp, s.n, flush=True)
+    o = Oracle(s.ncomp, s.comp_desc, s.mix, s.eps_rf, s.rc, s.rc_lj, s.box)
+    cot = np.random.default_rng(4).normal(size=(s.n, 3))
+    ref = o.adjoint(s.r, s.q, s.cid, cot)
+    with gpu_ctx(s) as c:
+        got = c.adjoint(cot)
+    for k in ("dU_deps", "dU_dsig", "dW_deps", "dW_dsig", "vjp"):
+        scale = max(np.abs(ref[k]).max(), 1e-300)
+        np.testing.assert_allclose(got[k], ref[k], rtol=0, atol=1e-11 * scale, err_msg=k)
+
+
 def test_adjoint_after_downloads():
     """Downloads and the adjoint share the context's staging buffer: repeated
     downloads and an adjoint after them keep working and agree."""
