@@ -28,7 +28,7 @@ typedef void (*AdjMolKernel)(int, const int *, const int *, const double *, cons
                              double *);
 typedef void (*GatherKernel)(int, const int *, const double *, double *);
 
-AdjKernel adjoint_kernel(int ti, int nlj);
+AdjKernel adjoint_kernel(int ti, int nlj, bool sites_at_centre);
 AdjMolKernel adjoint_mol_kernel();
 GatherKernel gather_orig3_kernel();
 

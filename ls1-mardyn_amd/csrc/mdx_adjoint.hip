@@ -10,18 +10,21 @@
 //  * the force vector-Jacobian product vjp_i = sum_j dF_j/dr_i . c_j with
 //    respect to molecule positions (orientations held fixed), for a
 //    cotangent c on the molecule forces.
-// Layout, as the forward pair kernels: one 64-lane block walks a strided set
-// of cells; the cell's i sites occupy nI lanes per phase and P = min(64 / nI,
-// 8) phases split the partners; the 27 neighbour images are staged into LDS
-// in chunks (site position and molecule position with the image shift
-// applied, the partner molecule's cotangent), a stage-A pass keeps per-lane
-// hit lists of molecule-centre distances below the cutoff and stage B
-// evaluates the derivative terms of the hits. Every site accumulates only its
-// own terms (owner computes), so each unordered pair is visited from both
-// sides: the parameter derivatives are halved per visit and summed over both
-// index orders on the host. Parameter sums go through per-lane LDS columns and
-// a fixed-order per-block fold, then a fixed-order reduction over blocks, so
-// results are deterministic.
+// Layout, as the forward pair kernels: one 64-lane block walks its share of
+// an XCD's contiguous range of cells; the cell's i sites occupy nI lanes per
+// phase and P = min(64 / nI, 8) phases split the partners; the 27 neighbour
+// images are staged into LDS in chunks (site position, and molecule position
+// where sites sit off their molecule's centre, with the image shift applied;
+// the partner molecule's cotangent), a stage-A pass keeps per-lane hit lists
+// of molecule-centre distances below the cutoff and stage B evaluates the
+// derivative terms of the hits. Every site accumulates only its own terms
+// (owner computes), so each unordered pair is visited from both sides: the
+// parameter derivatives are halved per visit and summed over both index
+// orders on the host. Parameter sums stay in registers (few LJ ids, charges)
+// or per-lane LDS columns until a fixed-order per-block fold, then a
+// fixed-order reduction over blocks, so results are deterministic.
+#include <cstddef>
+
 #include "mdx_adjoint.h"
 #include "mdx_pair_common.h"
 
@@ -31,9 +34,9 @@ constexpr int kAdjCap = 128;  // partner sites per LDS chunk
 constexpr int kAdjList = 16;  // per-lane hit list entries
 constexpr int kAdjUnroll = 4; // stage-A rows per drain check
 
-struct AdjTile {
+template <bool MS> struct AdjTile {
     Q4<double> p[kAdjCap];  // site position, i cell frame; w = LJ id or charge
-    Q4<double> m[kAdjCap];  // molecule position, i cell frame
+    Q4<double> m[MS ? 1 : kAdjCap];  // molecule position, i cell frame (MS: = p)
     Q4<double> c[kAdjCap];  // cotangent on the partner molecule's force
     unsigned short lst[kAdjList][64];
     int j0[27], pre[28];
@@ -43,17 +46,20 @@ struct AdjTile {
 // NLR > 0: the LJ parameter sums of a lane (partner LJ ids < NLR) stay in
 // registers until the chunk's fold instead of read-modify-writes of its LDS
 // column per pair (charge sites always keep their two sums in registers)
-template <int TI, int NLR>
+// MS: every site sits at its molecule's centre (one-centre LJ), so the tile
+// keeps no separate molecule positions
+template <int TI, int NLR, bool MS>
 __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
-    __shared__ AdjTile T;
+    __shared__ AdjTile<MS> T;
     extern __shared__ double lds[];
     // per-lane columns [W][64], the block table [K*W], lane keys [64], then
     // (LJ) the per-pair constants [nl * nl][kAdjPar]. Kernels that keep the
     // sums in registers fill the columns only for the fold after a chunk's
     // partner passes, when the tile is free: their columns use its bytes.
     constexpr bool ALIAS = TI != T_LJ || NLR > 0;
-    static_assert(!ALIAS || sizeof(AdjTile) >= (TI == T_LJ ? 4 * NLR : 2) * 64 * sizeof(double),
-                  "columns in the tile");
+    static_assert(!ALIAS || offsetof(AdjTile<MS>, lst) >=
+                                (TI == T_LJ ? 4 * NLR : 2) * 64 * sizeof(double),
+                  "columns in the tile's site records");
     const int W = a.W, K = a.K, TW = K * W;
     double *col = ALIAS ? reinterpret_cast<double *>(&T) : lds;
     double *table = ALIAS ? lds : lds + (size_t)W * 64;
@@ -165,7 +171,8 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                     const double sx = T.shift[k][0], sy = T.shift[k][1], sz = T.shift[k][2];
                     const double w = TI == T_LJ ? (double)S.ljid[j] : S.p[j];
                     T.p[t] = Q4<double>{S.x[j] + sx, S.y[j] + sy, S.z[j] + sz, w};
-                    T.m[t] = Q4<double>{S.mx[j] + sx, S.my[j] + sy, S.mz[j] + sz, 0.0};
+                    if constexpr (!MS)
+                        T.m[t] = Q4<double>{S.mx[j] + sx, S.my[j] + sy, S.mz[j] + sz, 0.0};
                     if (vjp) {
                         const int mj = S.mol[j];
                         T.c[t] = Q4<double>{a.cx[mj], a.cy[mj], a.cz[mj], 0.0};
@@ -181,7 +188,7 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                     for (int e = 0; e < emax; ++e) {
                         if (e >= cnt) continue;
                         const int t = T.lst[e][lane];
-                        const Q4<double> pj = T.p[t], mj4 = T.m[t];
+                        const Q4<double> pj = T.p[t], mj4 = MS ? pj : T.m[t];
                         const double dmx = mx - mj4.x, dmy = my - mj4.y, dmz = mz - mj4.z;
                         const double dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
                         const double r2 = dx * dx + dy * dy + dz * dz;
@@ -244,7 +251,7 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                 // kAdjUnroll rows per drain check; every candidate is written
                 // at the end of the list and kept only on a hit
                 auto hit = [&](int m) -> bool {
-                    const Q4<double> q = T.m[m];
+                    const Q4<double> q = MS ? T.p[m] : T.m[m];
                     const double ex = mx - q.x, ey = my - q.y, ez = mz - q.z;
                     const double r2m = ex * ex + ey * ey + ez * ez;
                     return r2m < rc2 && r2m != 0.0;
@@ -341,11 +348,15 @@ __global__ void k_gather_orig3(int n, const int *__restrict__ orig, const double
     out[2 * n + i] = in[3 * o + 2];
 }
 
-AdjKernel adjoint_kernel(int ti, int nlj) {
-    if (ti != T_LJ) return k_adjoint<T_C, 0>;
-    if (nlj <= 2) return k_adjoint<T_LJ, 2>;
-    if (nlj <= 4) return k_adjoint<T_LJ, 4>;
-    return k_adjoint<T_LJ, 0>;
+template <bool MS> AdjKernel adjoint_lj(int nlj) {
+    if (nlj <= 2) return k_adjoint<T_LJ, 2, MS>;
+    if (nlj <= 4) return k_adjoint<T_LJ, 4, MS>;
+    return k_adjoint<T_LJ, 0, MS>;
+}
+
+AdjKernel adjoint_kernel(int ti, int nlj, bool sites_at_centre) {
+    if (ti != T_LJ) return k_adjoint<T_C, 0, false>;
+    return sites_at_centre ? adjoint_lj<true>(nlj) : adjoint_lj<false>(nlj);
 }
 AdjMolKernel adjoint_mol_kernel() { return k_adjoint_mol; }
 GatherKernel gather_orig3_kernel() { return k_gather_orig3; }

@@ -1977,7 +1977,7 @@ int adjoint_run(mdx_context *c, const double *cot, double *dU_deps, double *dU_d
         const bool alias = t != T_LJ || nl <= 4;
         const size_t lds = ((alias ? 0 : (size_t)a.W * 64) + (size_t)a.K * a.W) * 8 + 64 * 4 +
                            (t == T_LJ ? (size_t)nl * nl * kAdjPar * 8 : 0);
-        hipLaunchKernelGGL(adjoint_kernel(t, nl), dim3(grid), dim3(64), lds, c->stream, a);
+        hipLaunchKernelGGL(adjoint_kernel(t, nl, t == T_LJ && c->lj_at_origin), dim3(grid), dim3(64), lds, c->stream, a);
         const int width = a.K * a.W;
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->stream, b.part, grid,
                            width, grid, t == T_LJ ? out_lj : out_q);
