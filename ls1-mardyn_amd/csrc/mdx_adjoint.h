@@ -29,6 +29,7 @@ typedef void (*AdjMolKernel)(int, const int *, const int *, const double *, cons
 typedef void (*GatherKernel)(int, const int *, const double *, double *);
 
 AdjKernel adjoint_kernel(int ti, int nlj, bool sites_at_centre);
+bool adjoint_columns_in_tile(int ti, int nlj);
 AdjMolKernel adjoint_mol_kernel();
 GatherKernel gather_orig3_kernel();
 

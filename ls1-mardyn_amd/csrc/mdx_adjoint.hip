@@ -354,6 +354,10 @@ template <bool MS> AdjKernel adjoint_lj(int nlj) {
     return k_adjoint<T_LJ, 0, MS>;
 }
 
+// Whether the kernel adjoint_kernel picks keeps its lane columns in its tile
+// (then the launch's dynamic LDS holds no columns).
+bool adjoint_columns_in_tile(int ti, int nlj) { return ti != T_LJ || nlj <= 4; }
+
 AdjKernel adjoint_kernel(int ti, int nlj, bool sites_at_centre) {
     if (ti != T_LJ) return k_adjoint<T_C, 0, false>;
     return sites_at_centre ? adjoint_lj<true>(nlj) : adjoint_lj<false>(nlj);

@@ -1974,7 +1974,7 @@ int adjoint_run(mdx_context *c, const double *cot, double *dU_deps, double *dU_d
         a.W = t == T_LJ ? 4 * nl : 2;
         // the kernels that keep their sums in registers (charges; LJ with at
         // most 4 ids) place the lane columns in their tile, not here
-        const bool alias = t != T_LJ || nl <= 4;
+        const bool alias = adjoint_columns_in_tile(t, nl);
         const size_t lds = ((alias ? 0 : (size_t)a.W * 64) + (size_t)a.K * a.W) * 8 + 64 * 4 +
                            (t == T_LJ ? (size_t)nl * nl * kAdjPar * 8 : 0);
         hipLaunchKernelGGL(adjoint_kernel(t, nl, t == T_LJ && c->lj_at_origin), dim3(grid), dim3(64), lds, c->stream, a);

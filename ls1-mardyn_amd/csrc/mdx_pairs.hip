@@ -556,13 +556,14 @@ __device__ __forceinline__ bool pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             }
         }
         for (int chunk = i_begin; chunk < i_end; chunk += 64) {
-            // lane = ph * nI + il: P = min(floor(64 / nI), 8) phases of the
-            // chunk's nI sites split the partner candidates; the 64 - P * nI
-            // lanes left over idle (phase 0, never a hit). Sparse cells have
-            // few candidates, and more than 8 phases would only lengthen the
-            // phase fold (measured)
+            // lane = ph * nI + il: P = floor(64 / nI) phases of the chunk's nI
+            // sites split the partner candidates; the 64 - P * nI lanes left
+            // over idle (phase 0, never a hit). LJ kernels cap P at 8: their
+            // sparse cells (vapour) have few candidates, and more phases only
+            // lengthen the phase fold; electrostatic kernels, whose cells hold
+            // a few sites of many candidates, gain from up to 16 (measured)
             const int nI = min(64, i_end - chunk);
-            const int P = min(64 / nI, 8);
+            const int P = min(64 / nI, TI == T_LJ ? 8 : 16);
             // lane / nI through a float reciprocal (exact: lane < 64, nI <= 64,
             // and (lane + 0.5) / nI stays >= 1/128 away from an integer)
             const int ph0 = (int)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)nI));
