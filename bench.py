@@ -237,6 +237,50 @@ def cpu_baseline(workload, budget_s):
                             "orientation only, not this host"})
 
 
+def run_step_variants(args):
+    """The step around the pair kernels in its ls1 variants, on the 1 M
+    mixture in the FP32 form (same state, same launch shape, one hipGraph per
+    step): plain NVE steps; NVT steps with the velocity-scaling thermostat and
+    the homogeneous long-range correction on the device; reflecting walls in
+    z; outflow walls in z (leaving molecules deleted, survivors compacted at
+    the next re-bin). Milliseconds per step for each, and the NVE pair-kernel
+    time for reference."""
+    from ls1amd import BOUNDARY_OUTFLOW, BOUNDARY_PERIODIC, BOUNDARY_REFLECTING
+
+    s = scenarios.mixture(n=int(1_000_000 * args.scale ** 3))
+    log(f"step variants: mixture N = {s.n}")
+    out = {"workload": "mixture step variants", "molecules": int(s.n), "steps": args.steps,
+           "precision": "FP32 pair / FP64 accumulate"}
+    variants = (
+        ("nve", None, None),
+        ("nvt_thermostat_lrc", s.temperature, None),
+        ("walls_reflecting_z", None, BOUNDARY_REFLECTING),
+        ("walls_outflow_z", None, BOUNDARY_OUTFLOW),
+    )
+    for name, T, wall in variants:
+        with Context(0) as ctx:
+            ctx.setup_system(s, periodic=True, precision=PREC_FP32_FP64ACC)
+            ctx.set_pair_config(*(PAIR_SHAPE or (1, 0)))
+            if T is not None:
+                ctx.set_thermostat(np.full(1, T), lrc=True)
+            if wall is not None:
+                ctx.set_boundaries([BOUNDARY_PERIODIC, BOUNDARY_PERIODIC, wall])
+            ctx.update_container()
+            ctx.traverse()
+            ctx.step(args.warmup, s.dt, use_graph=True)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.step(args.steps, s.dt, use_graph=True)
+            ctx.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / args.steps
+            rec = {"ms_per_step": ms, "molecule_steps_per_s": s.n * 1e3 / ms}
+            if wall == BOUNDARY_OUTFLOW:
+                rec["molecules_left"] = int(ctx.molecule_count())
+            out[name] = rec
+            log(f"  {name}: {ms:.3f} ms/step")
+    return out
+
+
 def run_replicas(args):
     """Batched replicas: all replicas in one context, one launch per kernel
     per step over the flattened cell table, one hipGraph per step, autotuned
@@ -498,6 +542,9 @@ def main():
         return
     if args.workload == "replicas":
         print(json.dumps(run_replicas(args)), flush=True)
+        return
+    if args.workload == "step-variants":
+        print(json.dumps(run_step_variants(args)), flush=True)
         return
 
     line = measure(args.workload, args.scale, args)
