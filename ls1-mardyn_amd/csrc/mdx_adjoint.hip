@@ -30,14 +30,20 @@
 
 namespace mdx {
 
-constexpr int kAdjCap = 128;  // partner sites per LDS chunk
 constexpr int kAdjList = 16;  // per-lane hit list entries
 constexpr int kAdjUnroll = 4; // stage-A rows per drain check
 
-template <bool MS> struct AdjTile {
-    Q4<double> p[kAdjCap];  // site position, i cell frame; w = LJ id or charge
-    Q4<double> m[MS ? 1 : kAdjCap];  // molecule position, i cell frame (MS: = p)
-    Q4<double> c[kAdjCap];  // cotangent on the partner molecule's force
+// Partner sites per LDS chunk: 96 keeps a one-centre block within 10 KB of
+// LDS, so 16 blocks (4 waves per SIMD) fit a CU's 160 KB and the registers
+// are allocated for that occupancy; the kernel with 4 LJ ids keeps 128, as
+// its sixteen register sums exceed the 4-wave register budget anyway and
+// its lane columns take the bytes of 128 site records.
+template <int TI, int NLR> constexpr int adj_cap() { return TI == T_LJ && NLR == 4 ? 128 : 96; }
+
+template <bool MS, int CAP> struct AdjTile {
+    Q4<double> p[CAP];  // site position, i cell frame; w = LJ id or charge
+    Q4<double> m[MS ? 1 : CAP];  // molecule position, i cell frame (MS: = p)
+    Q4<double> c[CAP];  // cotangent on the partner molecule's force
     unsigned short lst[kAdjList][64];
     int j0[27], pre[28];
     double shift[27][3];
@@ -50,14 +56,16 @@ template <bool MS> struct AdjTile {
 // keeps no separate molecule positions
 template <int TI, int NLR, bool MS>
 __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
-    __shared__ AdjTile<MS> T;
+    constexpr int CAP = adj_cap<TI, NLR>();
+    using Tile = AdjTile<MS, CAP>;
+    __shared__ Tile T;
     extern __shared__ double lds[];
     // per-lane columns [W][64], the block table [K*W], lane keys [64], then
     // (LJ) the per-pair constants [nl * nl][kAdjPar]. Kernels that keep the
     // sums in registers fill the columns only for the fold after a chunk's
     // partner passes, when the tile is free: their columns use its bytes.
     constexpr bool ALIAS = TI != T_LJ || NLR > 0;
-    static_assert(!ALIAS || offsetof(AdjTile<MS>, lst) >=
+    static_assert(!ALIAS || offsetof(Tile, lst) >=
                                 (TI == T_LJ ? 4 * NLR : 2) * 64 * sizeof(double),
                   "columns in the tile's site records");
     const int W = a.W, K = a.K, TW = K * W;
@@ -144,9 +152,11 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
             const bool valid = ph0 < P;
             const int ph = valid ? ph0 : 0;
             const int i = chunk + (valid ? il : 0);
-            const double px = S.x[i], py = S.y[i], pz = S.z[i];
-            // idle lanes test from far away and never hit
+            // idle lanes test from far away and never hit; one-centre sites
+            // (MS) sit at their molecule's position bit for bit (the site
+            // offset is a rotated zero vector)
             const double mx = valid ? S.mx[i] : 1e300, my = S.my[i], mz = S.mz[i];
+            const double px = MS ? mx : S.x[i], py = MS ? my : S.y[i], pz = MS ? mz : S.z[i];
             const int mi = S.mol[i];
             double ci[3] = {0, 0, 0};
             if (vjp) ci[0] = a.cx[mi], ci[1] = a.cy[mi], ci[2] = a.cz[mi];
@@ -162,8 +172,8 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
             constexpr int NR = TI == T_LJ ? (NLR > 0 ? 4 * NLR : 1) : 2;
             double racc[NR];
             for (int w = 0; w < NR; ++w) racc[w] = 0.0;
-            for (int base = 0; base < total; base += kAdjCap) {
-                const int nst = min(kAdjCap, total - base);
+            for (int base = 0; base < total; base += CAP) {
+                const int nst = min(CAP, total - base);
                 for (int t = lane; t < nst; t += 64) {
                     const int idx = base + t;
                     const int k = tile_k(T.pre, idx);
@@ -188,19 +198,18 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                     for (int e = 0; e < emax; ++e) {
                         if (e >= cnt) continue;
                         const int t = T.lst[e][lane];
-                        const Q4<double> pj = T.p[t], mj4 = MS ? pj : T.m[t];
-                        const double dmx = mx - mj4.x, dmy = my - mj4.y, dmz = mz - mj4.z;
+                        const Q4<double> pj = T.p[t];
                         const double dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
                         const double r2 = dx * dx + dy * dy + dz * dz;
                         const double r2i = Math<double>::rcp(r2);
-                        const double dmd = dmx * dx + dmy * dy + dmz * dz;
-                        double dc[3] = {0, 0, 0};
-                        if (vjp) {
-                            const Q4<double> cj = T.c[t];
-                            dc[0] = ci[0] - cj.x, dc[1] = ci[1] - cj.y, dc[2] = ci[2] - cj.z;
+                        // (m_i - m_j) . d: r^2 itself where sites are their
+                        // molecules' centres
+                        double dmd = r2;
+                        if constexpr (!MS) {
+                            const Q4<double> mj4 = T.m[t];
+                            dmd = (mx - mj4.x) * dx + (my - mj4.y) * dy + (mz - mj4.z) * dz;
                         }
-                        const double ddc = dx * dc[0] + dy * dc[1] + dz * dc[2];
-                        double sc, sp;
+                        double sc, sp;  // the force-derivative factors (VJP only)
                         if constexpr (TI == T_LJ) {
                             const int bj = (int)pj.w, kk = key * nl + bj;
                             const double *pt = ptab + (size_t)kk * kAdjPar;
@@ -217,11 +226,14 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                             if constexpr (NLR > 0) {
 #pragma unroll
                                 for (int b = 0; b < NLR; ++b) {
-                                    const bool mine = bj == b;
-                                    racc[4 * b] += mine ? 0.5 * gUe : 0.0;
-                                    racc[4 * b + 1] += mine ? 0.5 * gUs : 0.0;
-                                    racc[4 * b + 2] += mine ? 0.5 * gWe : 0.0;
-                                    racc[4 * b + 3] += mine ? 0.5 * gWs : 0.0;
+                                    // fma(h, x, s) with h = 0.5 or 0 rounds
+                                    // once, as s + 0.5 x does, and leaves the
+                                    // other ids' sums unchanged
+                                    const double h = bj == b ? 0.5 : 0.0;
+                                    racc[4 * b] = __builtin_fma(h, gUe, racc[4 * b]);
+                                    racc[4 * b + 1] = __builtin_fma(h, gUs, racc[4 * b + 1]);
+                                    racc[4 * b + 2] = __builtin_fma(h, gWe, racc[4 * b + 2]);
+                                    racc[4 * b + 3] = __builtin_fma(h, gWs, racc[4 * b + 3]);
                                 }
                             } else {
                                 double *c0 = col + (size_t)(bj * 4) * 64 + lane;
@@ -238,9 +250,14 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                             sc = qi * qj * r3i;
                             sp = -1.5 * qi * qj * r3i * r2i;
                         }
-                        g[0] += sc * dc[0] + 2.0 * sp * dx * ddc;
-                        g[1] += sc * dc[1] + 2.0 * sp * dy * ddc;
-                        g[2] += sc * dc[2] + 2.0 * sp * dz * ddc;
+                        if (vjp) {  // wave-uniform: the parameter-only call skips it
+                            const Q4<double> cj = T.c[t];
+                            const double dc[3] = {ci[0] - cj.x, ci[1] - cj.y, ci[2] - cj.z};
+                            const double ddc = dx * dc[0] + dy * dc[1] + dz * dc[2];
+                            g[0] += sc * dc[0] + 2.0 * sp * dx * ddc;
+                            g[1] += sc * dc[1] + 2.0 * sp * dy * ddc;
+                            g[2] += sc * dc[2] + 2.0 * sp * dz * ddc;
+                        }
                     }
                     cnt = 0;
                 };
@@ -284,12 +301,15 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
             }
             // the phases of each site into its phase-0 lane, in phase order
             // (every shuffle reads the lanes' own partial sums)
-            const double own[3] = {g[0], g[1], g[2]};
-            for (int q = 1; q < 8; ++q) {
-                const int src = min(il + q * nI, 63);
-                const double t0 = __shfl(own[0], src, 64), t1 = __shfl(own[1], src, 64),
-                             t2 = __shfl(own[2], src, 64);
-                if (q < P) g[0] += t0, g[1] += t1, g[2] += t2;
+            if (vjp) {
+                const double own[3] = {g[0], g[1], g[2]};
+                const int Ps = __builtin_amdgcn_readfirstlane(P);
+                for (int q = 1; q < Ps; ++q) {  // src < P nI <= 64
+                    const int src = il + q * nI;
+                    g[0] += __shfl(own[0], src, 64);
+                    g[1] += __shfl(own[1], src, 64);
+                    g[2] += __shfl(own[2], src, 64);
+                }
             }
             if (valid && ph == 0 && a.gx) {
                 a.gx[i] = g[0];
@@ -304,13 +324,32 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
             }
             lkey[lane] = valid ? key : -1;
             __syncthreads();
-            // fixed-order fold of the lane columns into the block table
-            for (int e = lane; e < TW; e += 64) {
-                const int kq = e / W, w = e - kq * W;
+            // fixed-order fold of the lane columns into the block table. With
+            // TW <= 32 outputs, output e = (key kq, column w) is summed by G
+            // lanes, each over 64 / G source lanes (G the largest power of
+            // two with G TW <= 64), and the G partial sums are added in lane
+            // order; otherwise one lane per output runs over all 64 lanes
+            if (TW <= 32) {
+                int G = 2;
+                while (2 * G * TW <= 64) G *= 2;
+                const int e = lane / G, part = lane - e * G, span = 64 / G;
                 double acc = 0.0;
-                for (int l = 0; l < 64; ++l)
-                    if (lkey[l] == kq) acc += col[w * 64 + l];
-                table[e] += acc;
+                if (e < TW) {
+                    const int kq = e / W, w = e - kq * W;
+                    for (int l = part * span; l < (part + 1) * span; ++l)
+                        if (lkey[l] == kq) acc += col[w * 64 + l];
+                }
+                double tot = acc;  // used by the first lane of each group
+                for (int q = 1; q < G; ++q) tot += __shfl(acc, min(lane + q, 63), 64);
+                if (e < TW && part == 0) table[e] += tot;
+            } else {
+                for (int e = lane; e < TW; e += 64) {
+                    const int kq = e / W, w = e - kq * W;
+                    double acc = 0.0;
+                    for (int l = 0; l < 64; ++l)
+                        if (lkey[l] == kq) acc += col[w * 64 + l];
+                    table[e] += acc;
+                }
             }
             __syncthreads();
             if constexpr (!ALIAS)

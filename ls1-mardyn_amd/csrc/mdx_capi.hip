@@ -32,10 +32,10 @@ __global__ void k_bin_gather(int, MolArrays, MolArrays, const int *, const int *
 __global__ void k_bin_gather_point(int, MolArrays, MolArrays, const int *, const int *,
                                    const int *, int *, const int *, int *, const Component *, Reps,
                                    float *, float *, float *, int *);
-__global__ void k_scan_block(const int *, int *, int, int *);
-__global__ void k_scan_add(int *, int, const int *);
-__global__ void k_scan_total(const int *, const int *, int, int *);
-__global__ void k_cell_site_start(int, const int *, const int *, int *);
+__global__ void k_scan_block(ScanArgs);
+__global__ void k_scan_add(ScanArgs);
+__global__ void k_scan_total(ScanArgs);
+__global__ void k_cell_site_start(int, const int *, ScanArgs);
 __global__ void k_build_sites(int, MolArrays, Reps, const int *, const Component *, SiteArrays,
                               SiteArrays, SiteArrays, SiteArrays);
 __global__ void k_calc_fm(int, MolArrays, const Component *, SiteArrays, SiteArrays, SiteArrays,
@@ -145,7 +145,12 @@ struct mdx_context {
     int *nsite_mol[kTypes] = {nullptr, nullptr, nullptr, nullptr};
     int *ident = nullptr;  // 0 .. n (point_lj: molecule index of each site, site starts)
     SiteArrays sites[kTypes]{};
-    std::vector<int *> scan_tmp;
+    // per scan level: kTypes segments of seg ints (block sums, then their scan)
+    struct ScanLevel {
+        int *p;
+        size_t seg;
+    };
+    std::vector<ScanLevel> scan_tmp;
     double *macro_part = nullptr, *lf_part = nullptr, *red = nullptr, *step_out = nullptr;
     int *step_idx = nullptr;
     int cap_steps = 0;
@@ -664,28 +669,39 @@ int alloc_scan(mdx_context *c, int n) {
     int len = std::max(n, 1);
     while (true) {
         int nb = (len + 1023) / 1024;
+        const size_t seg = (size_t)((nb + 1023) / 1024) * 1024 * 2 + 2048;
         int *p;
-        TRY(dalloc(c, &p, (size_t)((nb + 1023) / 1024) * 1024 * 2 + 2048, true));
-        c->scan_tmp.push_back(p);
+        TRY(dalloc(c, &p, seg * kTypes, true));
+        c->scan_tmp.push_back({p, seg});
         if (nb <= 1) break;
         len = nb;
     }
     return MDX_OK;
 }
 
-// Exclusive scan of n ints, out[n] = total, on the context stream.
-int scan_exclusive(mdx_context *c, const int *in, int *out, int n, int level = 0) {
-    int nb = std::max(1, (n + 1023) / 1024);
+// Exclusive scans of na arrays of a.n ints each in one set of launches
+// (array t on blockIdx.y), out_t[n] = total of in_t, on the context stream.
+int scan_exclusive(mdx_context *c, const ScanArgs &a, int na, int level = 0) {
+    const int n = a.n;
+    const int nb = std::max(1, (n + 1023) / 1024);
     if (level >= (int)c->scan_tmp.size()) return fail(c, MDX_ERR_CAPACITY, "scan depth");
-    int *bsum = c->scan_tmp[level];
-    hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(256), 0, c->stream, in, out, n, bsum);
-    if (nb > 1) {
+    const mdx_context::ScanLevel &L = c->scan_tmp[level];
+    ScanArgs blk = a, sub = a, add = a;
+    sub.n = nb;
+    for (int t = 0; t < na; ++t) {
+        int *bsum = L.p + (size_t)t * L.seg;
         int *bscan = bsum + ((nb + 1023) / 1024) * 1024;
-        TRY(scan_exclusive(c, bsum, bscan, nb, level + 1));
-        hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, c->stream, out, n, bscan);
+        blk.aux[t] = bsum;
+        sub.in[t] = bsum;
+        sub.out[t] = bscan;
+        add.aux[t] = bscan;
     }
-    if (level == 0)
-        hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, c->stream, in, out, n, out + n);
+    hipLaunchKernelGGL(k_scan_block, dim3(nb, na), dim3(256), 0, c->stream, blk);
+    if (nb > 1) {
+        TRY(scan_exclusive(c, sub, na, level + 1));
+        hipLaunchKernelGGL(k_scan_add, dim3(nb, na), dim3(256), 0, c->stream, add);
+    }
+    if (level == 0) hipLaunchKernelGGL(k_scan_total, dim3(1, na), dim3(1), 0, c->stream, a);
     return MDX_OK;
 }
 
@@ -758,7 +774,11 @@ int launch_update(mdx_context *c) {
                        c->cell, c->rank, c->count, c->ls1id);
     // the cells and the bin of deleted molecules behind them: start[ncell] is
     // the live count, start[ncell + 1] = n
-    TRY(scan_exclusive(c, c->count, c->start, c->ncell + 1));
+    ScanArgs cells{};
+    cells.in[0] = c->count;
+    cells.out[0] = c->start;
+    cells.n = c->ncell + 1;
+    TRY(scan_exclusive(c, cells, 1));
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, c->cell,
                        c->rank, c->start, c->perm);
     if (c->point_lj) {
@@ -776,11 +796,23 @@ int launch_update(mdx_context *c) {
                        c->nsite_mol[0],
                        c->nsite_mol[1], c->nsite_mol[2], c->nsite_mol[3]);
     c->cur = 1 - c->cur;
+    // per site type: molecule site starts (scan of the per-molecule site
+    // counts), then cell site starts, all present types in one launch each
+    ScanArgs ms{}, cs{};
+    int na = 0;
     for (int t = 0; t < kTypes; ++t) {
         if (c->nsites[t] == 0) continue;
-        TRY(scan_exclusive(c, c->nsite_mol[t], c->sites[t].mstart, n));
-        hipLaunchKernelGGL(k_cell_site_start, dim3(nblk(c->ncell + 1)), dim3(kBlock), 0, c->stream,
-                           c->ncell, c->start, c->sites[t].mstart, c->sites[t].cstart);
+        ms.in[na] = c->nsite_mol[t];
+        ms.out[na] = c->sites[t].mstart;
+        cs.in[na] = c->sites[t].mstart;
+        cs.out[na] = c->sites[t].cstart;
+        ++na;
+    }
+    ms.n = n;
+    if (na > 0) {
+        TRY(scan_exclusive(c, ms, na));
+        hipLaunchKernelGGL(k_cell_site_start, dim3(nblk(c->ncell + 1), na), dim3(kBlock), 0,
+                           c->stream, c->ncell, c->start, cs);
     }
     hipLaunchKernelGGL(k_build_sites, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, c->mol[c->cur],
                        reps(c), c->mcell, c->d_comps, c->sites[0], c->sites[1], c->sites[2],

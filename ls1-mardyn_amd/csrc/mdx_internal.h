@@ -76,6 +76,16 @@ struct SiteArrays {
     int *mstart;               // per molecule start (n + 1)
 };
 
+// Up to kTypes independent exclusive scans of n ints each, array t on
+// blockIdx.y = t; aux: the block sums (k_scan_block) or their exclusive scan
+// (k_scan_add).
+struct ScanArgs {
+    const int *in[kTypes];
+    int *out[kTypes];
+    int *aux[kTypes];
+    int n;
+};
+
 struct MolArrays {
     double *rx, *ry, *rz, *vx, *vy, *vz, *qw, *qx, *qy, *qz, *Lx, *Ly, *Lz;
     int *cid, *orig;

@@ -564,9 +564,11 @@ __global__ __launch_bounds__(256) void k_normalize_q(int n, MolArrays s, long lo
 // ------------------------------------------------------------- scan (int)
 
 // Exclusive scan of blocks of 1024 ints (256 threads x 4); block totals out.
-__global__ __launch_bounds__(256) void k_scan_block(const int *__restrict__ in, int *out, int n,
-                                                    int *blocksum) {
+__global__ __launch_bounds__(256) void k_scan_block(ScanArgs a) {
     __shared__ int ws[4];
+    const int *__restrict__ in = a.in[blockIdx.y];
+    int *out = a.out[blockIdx.y];
+    const int n = a.n;
     int base = blockIdx.x * 1024 + threadIdx.x * 4;
     int v[4], t = 0;
     for (int k = 0; k < 4; ++k) {
@@ -589,29 +591,34 @@ __global__ __launch_bounds__(256) void k_scan_block(const int *__restrict__ in, 
         if (base + k < n) out[base + k] = excl;
         excl += v[k];
     }
-    if (threadIdx.x == 255) blocksum[blockIdx.x] = woff + x;
+    if (threadIdx.x == 255) a.aux[blockIdx.y][blockIdx.x] = woff + x;
 }
 
-__global__ __launch_bounds__(256) void k_scan_add(int *out, int n, const int *__restrict__ off) {
+__global__ __launch_bounds__(256) void k_scan_add(ScanArgs a) {
+    int *out = a.out[blockIdx.y];
+    const int n = a.n;
     int i = blockIdx.x * 1024 + threadIdx.x;
-    int a = off[blockIdx.x];
+    int off = a.aux[blockIdx.y][blockIdx.x];
     for (int k = 0; k < 4; ++k) {
         int j = i + k * 256;
-        if (j < n) out[j] += a;
+        if (j < n) out[j] += off;
     }
 }
 
-__global__ void k_scan_total(const int *__restrict__ in, const int *__restrict__ out, int n,
-                             int *total) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) total[0] = (n > 0) ? out[n - 1] + in[n - 1] : 0;
+// out[n] = the total of array blockIdx.y (out holds n + 1 ints).
+__global__ void k_scan_total(ScanArgs a) {
+    const int *in = a.in[blockIdx.y];
+    int *out = a.out[blockIdx.y];
+    const int n = a.n;
+    if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = (n > 0) ? out[n - 1] + in[n - 1] : 0;
 }
 
-// cstart[c] = mstart[start[c]] for c in [0, ncell], start[ncell] = n.
-__global__ void k_cell_site_start(int ncell, const int *__restrict__ start,
-                                  const int *__restrict__ mstart, int *cstart) {
+// For array t = blockIdx.y: cstart_t[c] = mstart_t[start[c]] for c in
+// [0, ncell] (start[ncell] = n), with mstart_t = a.in[t], cstart_t = a.out[t].
+__global__ void k_cell_site_start(int ncell, const int *__restrict__ start, ScanArgs a) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c > ncell) return;
-    cstart[c] = mstart[start[c]];
+    a.out[blockIdx.y][c] = a.in[blockIdx.y][start[c]];
 }
 
 // --------------------------------------------------- SoA site cache build

@@ -317,9 +317,23 @@ template <int TI, typename R, bool MOLSITE, bool MULTI> struct TileCfg {
     // lists for their larger records.
     static constexpr int LIST = (sizeof(R) == 8 && !(TI == T_LJ && MOLSITE)) ? 16 : 24;
     static constexpr int CAP =
-        TI == T_LJ ? (MOLSITE ? (sizeof(R) == 4 ? 352 : 192) : (sizeof(R) == 4 ? 192 : 128))
-        : TI == T_C ? (sizeof(R) == 4 ? 112 : 96)
+        TI == T_LJ ? (MOLSITE ? (sizeof(R) == 4 ? 352 : 192) : (sizeof(R) == 4 ? 192 : 112))
+        : TI == T_C ? (sizeof(R) == 4 ? 112 : 72)
+        : TI == T_Q ? (sizeof(R) == 4 ? 120 : 104)
                     : (sizeof(R) == 4 ? 192 : 160);
+    // Minimum waves per SIMD the kernel's registers are allocated for (1: no
+    // bound, the compiler's choice). Tiles of at most 10 KB of LDS (lists
+    // included) let 16 blocks share a CU, 4 waves per SIMD, when the
+    // registers fit 128: the FP32 quadrupole kernel (120-site tile) and the
+    // FP64 charge kernel (72 sites) get the bound, which makes the compiler
+    // fit them with a few spilled values outside the pair loops; the FP64
+    // multi-site LJ kernel (112 sites) fits without it, and the FP64
+    // quadrupole kernel's 104-site tile (13 KB) reaches 3 waves per SIMD
+    // with its 162 registers. Measured on the mixture against the larger
+    // tiles at fewer waves: FP32 quadrupole kernel -6 %; FP64 LJ -3 %,
+    // charge -4 %, quadrupole -9 %.
+    static constexpr int WAVES =
+        (TI == T_Q && sizeof(R) == 4) || (TI == T_C && sizeof(R) == 8) ? 4 : 1;
 };
 
 template <int TI, typename R, bool MOLSITE, bool MULTI> struct Tile {

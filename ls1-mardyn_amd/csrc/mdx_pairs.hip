@@ -683,7 +683,8 @@ __device__ __forceinline__ void flush_macro(const PairArgs &a, double (&wmac)[3]
 // persistent: each walks its XCD's items with stride gridDim.x / 8, and every
 // block leaves after its last item.
 template <int TI, typename R, bool COUNT, bool MOLSITE, bool MULTI>
-__global__ __launch_bounds__(64) void k_pairs(PairArgs a) {
+__global__ __launch_bounds__(64) __attribute__((
+    amdgpu_waves_per_eu(TileCfg<TI, R, MOLSITE, MULTI>::WAVES))) void k_pairs(PairArgs a) {
     __shared__ Tile<TI, R, MOLSITE, MULTI> T;
     const int lane = threadIdx.x;
     const int xcd = blockIdx.x & 7, stride = gridDim.x >> 3;
