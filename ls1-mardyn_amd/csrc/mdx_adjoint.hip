@@ -33,12 +33,14 @@ namespace mdx {
 constexpr int kAdjList = 16;  // per-lane hit list entries
 constexpr int kAdjUnroll = 4; // stage-A rows per drain check
 
-// Partner sites per LDS chunk: 96 keeps a one-centre block within 10 KB of
-// LDS, so 16 blocks (4 waves per SIMD) fit a CU's 160 KB and the registers
-// are allocated for that occupancy; the kernel with 4 LJ ids keeps 128, as
-// its sixteen register sums exceed the 4-wave register budget anyway and
-// its lane columns take the bytes of 128 site records.
-template <int TI, int NLR> constexpr int adj_cap() { return TI == T_LJ && NLR == 4 ? 128 : 96; }
+// Partner sites per LDS chunk: 104 keeps a one-centre block within 10 KB of
+// LDS (9.4 KB of tile and lists, 0.6 KB of tables for up to 2 LJ ids), so 16
+// blocks (4 waves per SIMD) fit a CU's 160 KB and the registers are
+// allocated for that occupancy, and most cells' 27 neighbour images fit 3
+// chunks; the kernel with 4 LJ ids keeps 128, as its sixteen register sums
+// exceed the 4-wave register budget anyway and its lane columns take the
+// bytes of 128 site records.
+template <int TI, int NLR> constexpr int adj_cap() { return TI == T_LJ && NLR == 4 ? 128 : 104; }
 
 template <bool MS, int CAP> struct AdjTile {
     Q4<double> p[CAP];  // site position, i cell frame; w = LJ id or charge
