@@ -117,7 +117,7 @@ int mdx_set_replicas(mdx_context *ctx, int nrep, const double *boxes, const int6
 int mdx_set_pair_config(mdx_context *ctx, int cells_per_block, int blocks_per_xcd);
 
 /* Times the pair kernels of the current state over the candidate launch
- * shapes (cells_per_block 1, 2, 4, 8 x blocks_per_xcd 0, 64, 128, 256, 512),
+ * shapes (cells_per_block 1, 2, 4, 8 x blocks_per_xcd 0, 512, 1024),
  * reps timed traversals each (HIP events; a shape's time is its fastest
  * run), and keeps the fastest shape. chosen (may be NULL) =
  * {cells_per_block, blocks_per_xcd}; best_ms = its time. */

@@ -331,7 +331,7 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
             // lanes, each over 64 / G source lanes (G the largest power of
             // two with G TW <= 64), and the G partial sums are added in lane
             // order; otherwise one lane per output runs over all 64 lanes
-            if (TW <= 32) {
+            if (TW > 0 && TW <= 32) {
                 int G = 2;
                 while (2 * G * TW <= 64) G *= 2;
                 const int e = lane / G, part = lane - e * G, span = 64 / G;
