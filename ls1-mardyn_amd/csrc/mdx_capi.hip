@@ -655,6 +655,7 @@ int alloc_sites(mdx_context *c, SiteArrays &s, int cap, int ncell, int n, int ty
         TRY(dalloc(c, &s.p, cap, true));
         TRY(dalloc(c, &s.fp, cap, true));
     }
+    if (type == T_C) TRY(dalloc(c, &s.nhead, cap, true));
     if (type >= T_D) {
         double **ep[] = {&s.ex, &s.ey, &s.ez, &s.Mx, &s.My, &s.Mz};
         for (double **p : ep) TRY(dalloc(c, p, cap, true));

@@ -69,6 +69,8 @@ struct SiteArrays {
     double *p;                 // C: q, D: |mu|, Q: |Q|
     float *fp;
     int *ljid;                 // LJ lookup id
+    int *nhead;                // charges: the molecule's charge count at its first
+                               // charge site, 0 at the others (per-molecule tests)
     double *Fx, *Fy, *Fz;      // outputs: force
     double *Vx, *Vy, *Vz;      // virial
     double *Mx, *My, *Mz;      // torque (D, Q)

@@ -671,6 +671,7 @@ __global__ __launch_bounds__(256) void k_build_sites(int n, MolArrays s, Reps R,
                 A.p[j] = c.param[t][k];
                 A.fp[j] = (float)c.param[t][k];
             }
+            if (t == T_C) A.nhead[j] = k == 0 ? c.n[T_C] : 0;
             if (t >= T_D) {
                 double e[3];
                 qrot(q, c.e[t][k], e);

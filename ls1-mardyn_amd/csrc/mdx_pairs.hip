@@ -259,11 +259,11 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
     using PT = PassTile<TI, TJ, R, MOLSITE, MULTI>;
     static_assert(PT::HEADS == heads, "head layout");
     const PT V(T);
+    // (a molecule's charges are contiguous, and its first one carries the
+    // count: one load per site, no chain through the molecule index)
     auto is_head = [&](int idx) -> bool {
         const int k = tile_k(T.pre, idx);
-        if (idx == T.pre[k]) return true;  // a cell range starts with a molecule
-        const int j = T.j0[k] + (idx - T.pre[k]);
-        return J.mol[j - 1] != J.mol[j];
+        return J.nhead[T.j0[k] + (idx - T.pre[k])] != 0;
     };
     for (int base = 0; base < total;) {
         int nst = min(PT::CAP, total - base);
@@ -303,15 +303,8 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             if constexpr (C::MOL) {
                 const V3<R> m = SiteView<R>::mol(J, j);
                 if constexpr (heads) {
-                    head = idx == T.pre[k];
-                    if (!head) head = J.mol[j - 1] != J.mol[j];
-                }
-                if constexpr (heads) {
-                    int w = 0;
-                    if (head) {
-                        const int mo = J.mol[j];
-                        w = J.mstart[mo + 1] - J.mstart[mo];
-                    }
+                    const int w = J.nhead[j];
+                    head = w != 0;
                     V.ns[s] = (unsigned char)w;
                 }
                 Q4<R> mq{m.x + off.x, m.y + off.y, m.z + off.z, R(0)};
