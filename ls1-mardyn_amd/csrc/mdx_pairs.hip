@@ -261,22 +261,22 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
     const PT V(T);
     // (a molecule's charges are contiguous, and its first one carries the
     // count: one load per site, no chain through the molecule index)
-    auto is_head = [&](int idx) -> bool {
-        const int k = tile_k(T.pre, idx);
-        return J.nhead[T.j0[k] + (idx - T.pre[k])] != 0;
-    };
     for (int base = 0; base < total;) {
         int nst = min(PT::CAP, total - base);
+        // heads: a chunk that ends inside a molecule ends at that molecule's
+        // first site instead (the last head of the chunk); whether it does is
+        // the head count of the site after the chunk, loaded beside the
+        // chunk's own sites
+        int next_head = 1;
         if constexpr (heads) {
-            if (base + nst < total) {  // back off to the first site of the split molecule
-                const int t = lane;
-                const bool h = t < kMaxSite && t < nst && is_head(base + nst - t);
-                const unsigned long long hm = __ballot(h);
-                if (hm) nst -= __ffsll((long long)hm) - 1;
+            if (base + nst < total) {
+                const int idx = base + nst;
+                const int k = tile_k(T.pre, idx);
+                next_head = J.nhead[T.j0[k] + (idx - T.pre[k])];
             }
         }
         int nh = nst;
-        int hb = 0;
+        int hb = 0, last_head = 0;
         for (int s0 = 0; s0 < nst; s0 += 64) {
             const int s = s0 + lane;
             bool head = false;
@@ -320,9 +320,20 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
                 const unsigned long long hm = __ballot(head);
                 if (head) V.h[hb + __popcll(hm & ((1ull << lane) - 1ull))] = (unsigned short)s;
                 hb += __popcll(hm);
+                if (hm) last_head = s0 + 63 - __clzll((long long)hm);
             }
         }
-        if constexpr (heads) nh = hb;
+        if constexpr (heads) {
+            nh = hb;
+            // the split molecule's sites run from the last head to the chunk
+            // end: drop them (and that head), the next chunk starts there
+            // (a chunk holds many molecules of at most kMaxSite sites, so
+            // last_head > 0)
+            if (next_head == 0 && last_head > 0) {
+                nst = last_head;
+                nh = hb - 1;
+            }
+        }
         __syncthreads();
         int cnt = 0;
         // Stage B. A listed candidate is final unless its molecule distance
