@@ -20,6 +20,7 @@ struct AdjArgs {
     double *part;              // [blocks][K * W] parameter partials
     int nlj, ncell;
     int K, W;                  // keys per block table, values per key
+    int site_is_mol;           // site j belongs to molecule j (one-centre systems)
 };
 
 typedef void (*AdjKernel)(AdjArgs);

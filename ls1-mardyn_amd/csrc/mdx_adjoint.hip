@@ -159,7 +159,7 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
             // offset is a rotated zero vector)
             const double mx = valid ? S.mx[i] : 1e300, my = S.my[i], mz = S.mz[i];
             const double px = MS ? mx : S.x[i], py = MS ? my : S.y[i], pz = MS ? mz : S.z[i];
-            const int mi = S.mol[i];
+            const int mi = a.site_is_mol ? i : S.mol[i];
             double ci[3] = {0, 0, 0};
             if (vjp) ci[0] = a.cx[mi], ci[1] = a.cy[mi], ci[2] = a.cz[mi];
             int key;
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(64) void k_adjoint(AdjArgs a) {
                     if constexpr (!MS)
                         T.m[t] = Q4<double>{S.mx[j] + sx, S.my[j] + sy, S.mz[j] + sz, 0.0};
                     if (vjp) {
-                        const int mj = S.mol[j];
+                        const int mj = a.site_is_mol ? j : S.mol[j];
                         T.c[t] = Q4<double>{a.cx[mj], a.cy[mj], a.cz[mj], 0.0};
                     }
                 }

@@ -2006,6 +2006,7 @@ int adjoint_run(mdx_context *c, const double *cot, double *dU_deps, double *dU_d
         a.part = b.part;
         a.nlj = nl;
         a.ncell = c->ncell;
+        a.site_is_mol = c->point_lj && t == T_LJ;  // site_view: mol is the identity
         a.K = t == T_LJ ? nl : ncomp * kMaxSite;
         a.W = t == T_LJ ? 4 * nl : 2;
         // the kernels that keep their sums in registers (charges; LJ with at
