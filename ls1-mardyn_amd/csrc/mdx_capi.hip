@@ -36,10 +36,11 @@ __global__ void k_scan_block(ScanArgs);
 __global__ void k_scan_add(ScanArgs);
 __global__ void k_scan_total(ScanArgs);
 __global__ void k_cell_site_start(int, const int *, ScanArgs);
-__global__ void k_build_sites(int, MolArrays, Reps, const int *, const Component *, SiteArrays,
-                              SiteArrays, SiteArrays, SiteArrays);
-__global__ void k_calc_fm(int, MolArrays, const Component *, SiteArrays, SiteArrays, SiteArrays,
-                          SiteArrays, double *, double *, double *, Reps, const double *);
+__global__ void k_build_sites(int, MolArrays, Reps, const int *, const Component *, int,
+                              SiteArrays, SiteArrays, SiteArrays, SiteArrays);
+__global__ void k_calc_fm(int, MolArrays, const Component *, int, SiteArrays, SiteArrays,
+                          SiteArrays, SiteArrays, double *, double *, double *, Reps,
+                          const double *);
 __global__ void k_velocity_scale(MolArrays, double *, double *, const Component *, Reps,
                                  const double *, int);
 __global__ void k_scatter_orig(int, const int *, const double *, double *, int, int);
@@ -815,9 +816,10 @@ int launch_update(mdx_context *c) {
         hipLaunchKernelGGL(k_cell_site_start, dim3(nblk(c->ncell + 1), na), dim3(kBlock), 0,
                            c->stream, c->ncell, c->start, cs);
     }
-    hipLaunchKernelGGL(k_build_sites, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, c->mol[c->cur],
-                       reps(c), c->mcell, c->d_comps, c->sites[0], c->sites[1], c->sites[2],
-                       c->sites[3]);
+    const int nc = c->model.ncomp;
+    hipLaunchKernelGGL(k_build_sites, dim3(nblk(n)), dim3(kBlock), nc * sizeof(Component),
+                       c->stream, n, c->mol[c->cur], reps(c), c->mcell, c->d_comps, nc,
+                       c->sites[0], c->sites[1], c->sites[2], c->sites[3]);
     c->binned = true;
     HIPCHK(c, hipGetLastError());
     return MDX_OK;
@@ -897,9 +899,10 @@ int launch_traverse(mdx_context *c, double kick = 0.0) {
                            c->d_cell_off, c->red);
     // point_lj: the pair kernel wrote F and Vi; M stays 0 (no site offsets)
     if (!c->point_lj)
-        hipLaunchKernelGGL(k_calc_fm, dim3(nblk(c->n)), dim3(kBlock), 0, c->stream, c->n,
-                           c->mol[c->cur], c->d_comps, c->sites[0], c->sites[1], c->sites[2],
-                           c->sites[3], c->F, c->M, c->Vi, reps(c),
+        hipLaunchKernelGGL(k_calc_fm, dim3(nblk(c->n)), dim3(kBlock),
+                           c->model.ncomp * sizeof(Component), c->stream, c->n,
+                           c->mol[c->cur], c->d_comps, c->model.ncomp, c->sites[0],
+                           c->sites[1], c->sites[2], c->sites[3], c->F, c->M, c->Vi, reps(c),
                            c->lrc ? c->d_thermo : nullptr);
     hipLaunchKernelGGL(k_finalize_macro, dim3((R + 63) / 64), dim3(64), 0, c->stream, c->red,
                        c->red + 4 * R, c->step_out, c->step_idx, R,

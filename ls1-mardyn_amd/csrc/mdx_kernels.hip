@@ -623,15 +623,31 @@ __global__ void k_cell_site_start(int ncell, const int *__restrict__ start, Scan
 
 // --------------------------------------------------- SoA site cache build
 
+// The block's copy of the component table (ncomp records) in LDS: the site
+// loops read offsets, axes and parameters per site, and a component's record
+// is then one LDS read away instead of a global load after the cid load.
+// Every thread of the block must call it (it synchronises the block).
+__device__ __forceinline__ const Component *stage_comps(const Component *__restrict__ comps,
+                                                        int ncomp) {
+    extern __shared__ double comp_lds[];
+    static_assert(sizeof(Component) % sizeof(double) == 0, "component record size");
+    const double *src = reinterpret_cast<const double *>(comps);
+    const int nw = ncomp * (int)(sizeof(Component) / sizeof(double));
+    for (int w = threadIdx.x; w < nw; w += blockDim.x) comp_lds[w] = src[w];
+    __syncthreads();
+    return reinterpret_cast<const Component *>(comp_lds);
+}
+
 // FullMolecule::setupSoACache (FullMolecule.cpp:711-767): normalise q (the
 // reference mutates the state here, Quaternion.h:40-42), rotate the body-frame
 // sites, record molecule position and parameters; positions also relative to
 // the molecule's cell corner in FP32.
 __global__ __launch_bounds__(256) void k_build_sites(int n, MolArrays s, Reps R,
                                                      const int *__restrict__ mcell,
-                                                     const Component *__restrict__ comps,
-                                                     SiteArrays s0, SiteArrays s1, SiteArrays s2,
-                                                     SiteArrays s3) {
+                                                     const Component *__restrict__ gcomps,
+                                                     int ncomp, SiteArrays s0, SiteArrays s1,
+                                                     SiteArrays s2, SiteArrays s3) {
+    const Component *comps = stage_comps(gcomps, ncomp);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
@@ -685,10 +701,12 @@ __global__ __launch_bounds__(256) void k_build_sites(int n, MolArrays s, Reps R,
 // FullMolecule::calcFM (FullMolecule.cpp:548-631): F = sum F_site,
 // M = sum d_site x F_site + sum M_site(D, Q), Vi = 0.5 sum V_site.
 __global__ __launch_bounds__(256) void k_calc_fm(int n, MolArrays s,
-                                                 const Component *__restrict__ comps,
-                                                 SiteArrays s0, SiteArrays s1, SiteArrays s2,
-                                                 SiteArrays s3, double *F, double *M, double *Vi,
-                                                 Reps R, const double *__restrict__ thermo) {
+                                                 const Component *__restrict__ gcomps,
+                                                 int ncomp, SiteArrays s0, SiteArrays s1,
+                                                 SiteArrays s2, SiteArrays s3, double *F,
+                                                 double *M, double *Vi, Reps R,
+                                                 const double *__restrict__ thermo) {
+    const Component *comps = stage_comps(gcomps, ncomp);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Component &c = comps[s.cid[i]];
