@@ -709,8 +709,13 @@ int scan_exclusive(mdx_context *c, const ScanArgs &a, int na, int level = 0) {
 
 // Work items of cpb consecutive cells in 8 contiguous ranges (one per XCD);
 // one block per item, or blocks_per_xcd persistent blocks per XCD.
-int pair_grid(const mdx_context *c, int *items_per_xcd) {
-    const int items = (c->ncell + c->cpb - 1) / c->cpb;
+// The quadrupole kernel takes cells two at a time, so its items are twice as
+// long (same number of cell pairs per item as single cells per item).
+int cells_per_item(const mdx_context *c, int t) { return t == T_Q ? 2 * c->cpb : c->cpb; }
+
+int pair_grid(const mdx_context *c, int *items_per_xcd, int t = T_LJ) {
+    const int cpb = cells_per_item(c, t);
+    const int items = (c->ncell + cpb - 1) / cpb;
     *items_per_xcd = (items + 7) / 8;
     int per = *items_per_xcd;
     if (c->blocks_per_xcd > 0) per = std::min(per, c->blocks_per_xcd);
@@ -864,8 +869,6 @@ int launch_traverse(mdx_context *c, double kick = 0.0) {
     a.pp.fpar = c->d_fpar;
     a.pp.nlj = c->model.nlj;
     a.ncell = c->ncell;
-    a.cpb = c->cpb;
-    int grid = pair_grid(c, &a.items_per_xcd);
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     // one [ncell][4] partial array: the first i-site type writes each cell's
     // row, later types add to it (the launches are ordered on the stream)
@@ -875,6 +878,8 @@ int launch_traverse(mdx_context *c, double kick = 0.0) {
         a.macro_part = c->macro_part;
         a.accumulate = slot > 0;
         a.counters = c->d_counters + 8 * t;  // each kernel its own row
+        a.cpb = cells_per_item(c, t);
+        const int grid = pair_grid(c, &a.items_per_xcd, t);
         PairKernel k = pair_kernel(t, c->precision == MDX_PREC_FP32_FP64ACC, c->counting,
                                    t == T_LJ && c->lj_at_origin, c->model.nlj > 1);
         if (c->timing) {

@@ -223,11 +223,13 @@ struct Nbr {
     int dd[3];
 };
 
-__device__ __forceinline__ Nbr neighbour(const Geom &g, int cx, int cy, int cz, int k) {
+// nxs = 4 (two cells side by side in x): x offsets -1 .. 2 from the first.
+__device__ __forceinline__ Nbr neighbour(const Geom &g, int cx, int cy, int cz, int k,
+                                         int nxs = 3) {
     Nbr b;
-    b.dd[0] = k % 3 - 1;
-    b.dd[1] = (k / 3) % 3 - 1;
-    b.dd[2] = k / 9 - 1;
+    b.dd[0] = k % nxs - 1;
+    b.dd[1] = (k / nxs) % 3 - 1;
+    b.dd[2] = k / (3 * nxs) - 1;
     int c3[3] = {cx + b.dd[0], cy + b.dd[1], cz + b.dd[2]};
     b.ok = true;
     for (int d = 0; d < 3; ++d) {
@@ -247,9 +249,9 @@ __device__ __forceinline__ Nbr neighbour(const Geom &g, int cx, int cy, int cz, 
 }
 
 // Neighbour index of tile entry idx (concatenated neighbour order): the
-// largest k with pre[k] <= idx.
-__device__ __forceinline__ int tile_k(const int *pre, int idx) {
-    int lo = 0, hi = 27;
+// largest k < nb with pre[k] <= idx.
+__device__ __forceinline__ int tile_k(const int *pre, int idx, int nb = 27) {
+    int lo = 0, hi = nb;
     while (hi - lo > 1) {
         int mid = (lo + hi) >> 1;
         if (pre[mid] <= idx) lo = mid; else hi = mid;
@@ -319,17 +321,25 @@ template <int TI, typename R, bool MOLSITE, bool MULTI> struct TileCfg {
     static constexpr int CAP =
         TI == T_LJ ? (MOLSITE ? (sizeof(R) == 4 ? 352 : 192) : (sizeof(R) == 4 ? 192 : 112))
         : TI == T_C ? (sizeof(R) == 4 ? 112 : 72)
-        : TI == T_Q ? (sizeof(R) == 4 ? 120 : 104)
+        : TI == T_Q ? (sizeof(R) == 4 ? 114 : 104)
                     : (sizeof(R) == 4 ? 192 : 160);
+    // Quadrupole i sites are few per cell (one per 2CLJQ molecule) against
+    // a neighbourhood of many partner sites, so the kernel takes two cells
+    // side by side in x at a time (JOINT): their neighbourhood of 4 x 3 x 3
+    // cells is staged once for both, two thirds of what two 27-cell tiles
+    // load, at the price of tests against the 9 cells that are beyond the
+    // cutoff of one of the two (NB neighbour ranges at most).
+    static constexpr bool JOINT = TI == T_Q;
+    static constexpr int NB = JOINT ? 36 : 27;
     // Minimum waves per SIMD the kernel's registers are allocated for (1: no
     // bound, the compiler's choice). Tiles of at most 10 KB of LDS (lists
     // included) let 16 blocks share a CU, 4 waves per SIMD, when the
-    // registers fit 128: the FP32 quadrupole kernel (120-site tile) and the
+    // registers fit 128: the FP32 quadrupole kernel (114-site tile) and the
     // FP64 charge kernel (72 sites) get the bound, which makes the compiler
     // fit them with a few spilled values outside the pair loops; the FP64
     // multi-site LJ kernel (112 sites) fits without it, and the FP64
     // quadrupole kernel's 104-site tile (13 KB) reaches 3 waves per SIMD
-    // with its 162 registers. Measured on the mixture against the larger
+    // with its 166 registers. Measured on the mixture against the larger
     // tiles at fewer waves: FP32 quadrupole kernel -6 %; FP64 LJ -3 %,
     // charge -4 %, quadrupole -9 %.
     static constexpr int WAVES =
@@ -346,8 +356,8 @@ template <int TI, typename R, bool MOLSITE, bool MULTI> struct Tile {
         (int)sizeof(Q4<R>) * (1 + C::MOL + C::ORI) + (C::MOL && TI != T_LJ ? 3 : 0);
     alignas(32) unsigned char raw[C::CAP * SLOT];
     unsigned short lst[C::LIST][64];  // per-lane hit lists (tile indices)
-    int j0[27], pre[28];
-    double shift[27][3];
+    int j0[C::NB], pre[C::NB + 1];
+    double shift[C::NB][3];
 };
 
 // The records of one partner pass (partner type TJ) in a tile's slot bytes:
@@ -377,16 +387,18 @@ template <int TI, int TJ, typename R, bool MOLSITE, bool MULTI> struct PassTile 
 };
 
 template <typename R>
-__device__ __forceinline__ V3<R> tile_offset(const Geom &g, const double (*shift)[3], int k);
+__device__ __forceinline__ V3<R> tile_offset(const Geom &g, const double (*shift)[3], int k,
+                                             int nxs);
 template <>
 __device__ __forceinline__ V3<double> tile_offset<double>(const Geom &, const double (*shift)[3],
-                                                          int k) {
+                                                          int k, int) {
     return {shift[k][0], shift[k][1], shift[k][2]};
 }
 template <>
-__device__ __forceinline__ V3<float> tile_offset<float>(const Geom &g, const double (*)[3], int k) {
-    return {(float)((k % 3 - 1) * g.cl[0]), (float)((k / 3 % 3 - 1) * g.cl[1]),
-            (float)((k / 9 - 1) * g.cl[2])};
+__device__ __forceinline__ V3<float> tile_offset<float>(const Geom &g, const double (*)[3], int k,
+                                                        int nxs) {
+    return {(float)((k % nxs - 1) * g.cl[0]), (float)((k / nxs % 3 - 1) * g.cl[1]),
+            (float)((k / (3 * nxs) - 1) * g.cl[2])};
 }
 
 }  // namespace mdx

@@ -198,7 +198,8 @@ constexpr int kUnroll = 4;  // stage-A candidates in flight per lane
 template <int TI, int TJ, typename R, bool MOLSITE, bool MULTI, bool COUNT>
 __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, int cbase,
                                              const ISite<R> &I, bool valid,
-                                             int cx, int cy, int cz, int ph, int P, int lane,
+                                             int cx, int cy, int cz, int nxs, int ph, int P,
+                                             int lane,
                                              Tile<TI, R, MOLSITE, MULTI> &T, Acc &acc, R eps24,
                                              R sig2, R shift6, unsigned long long &ntest) {
     using C = TileCfg<TI, R, MOLSITE, MULTI>;
@@ -221,14 +222,18 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
     // by the rounding bound of the expanded r^2 where that is used
     R cut = sizeof(R) == 4 ? hi : rc2r;
     if constexpr (NORM) {
+        // (the bound grows with the square of the frame's extent: nxs - 1
+        // cell edges in x when two cells share the tile)
         const double clm = fmax(g.cl[0], fmax(g.cl[1], g.cl[2]));
-        const double band = fmax(1e-5, 1.5e-5 * clm * clm / rc2);
+        const double ext = 0.5 * (nxs - 1);
+        const double band = fmax(1e-5, 1.5e-5 * ext * ext * clm * clm / rc2);
         cut = (R)(rc2 * (1.0 + band));
     }
     unsigned long long nhit = 0;
     int nk = 0;
-    if (lane < 27) {
-        Nbr b = neighbour(g, cx, cy, cz, lane);
+    const int nb = 9 * nxs;  // neighbour ranges: 27, or 36 for two cells
+    if (lane < nb) {
+        Nbr b = neighbour(g, cx, cy, cz, lane, nxs);
         int j0 = 0;
         if (b.ok) {
             j0 = J.cstart[cbase + b.cell];
@@ -240,13 +245,13 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         T.shift[lane][2] = b.s[2];
     }
     int incl = nk;
-    for (int o = 1; o < 32; o <<= 1) {
+    for (int o = 1; o < (C::NB > 32 ? 64 : 32); o <<= 1) {
         int y = __shfl_up(incl, o, 64);
         if (lane >= o) incl += y;
     }
-    if (lane < 27) T.pre[lane + 1] = incl;
+    if (lane < nb) T.pre[lane + 1] = incl;
     if (lane == 0) T.pre[0] = 0;
-    const int total = __shfl(incl, 26, 64);
+    const int total = __shfl(incl, nb - 1, 64);
     __syncthreads();
     // Charge partners are tested once per molecule (neutral molecules carry
     // two or more charges): the tile keeps a molecule's sites contiguous, the
@@ -271,7 +276,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         if constexpr (heads) {
             if (base + nst < total) {
                 const int idx = base + nst;
-                const int k = tile_k(T.pre, idx);
+                const int k = tile_k(T.pre, idx, nb);
                 next_head = J.nhead[T.j0[k] + (idx - T.pre[k])];
             }
         }
@@ -282,9 +287,9 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             bool head = false;
             if (s < nst) {
             const int idx = base + s;
-            const int k = tile_k(T.pre, idx);
+            const int k = tile_k(T.pre, idx, nb);
             const int j = T.j0[k] + (idx - T.pre[k]);
-            const V3<R> off = tile_offset<R>(g, T.shift, k);
+            const V3<R> off = tile_offset<R>(g, T.shift, k, nxs);
             const V3<R> p = SiteView<R>::pos(J, j);
             Q4<R> q;
             q.x = p.x + off.x;
@@ -342,7 +347,7 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
         // from the unrounded positions so the pair set equals the FP64 one.
         auto recheck = [&](int s) -> bool {  // the FP64 decision for tile entry s
             const int idx = base + s;
-            const int k = tile_k(T.pre, idx);
+            const int k = tile_k(T.pre, idx, nb);
             const int j = T.j0[k] + (idx - T.pre[k]);
             const SiteArrays &S = a.s[TI];
             const double ex = S.mx[I.idx] - (J.mx[j] + T.shift[k][0]);
@@ -536,18 +541,22 @@ __device__ __forceinline__ double shfl_xor_d(double v, int o) { return __shfl_xo
 // All i sites of one cell (global index) with their partners; adds the
 // cell's per-lane macroscopic sums to wmac and writes a zero row for the cell
 // (flush_macro later stores the sums of a group of cells in one row).
-// Returns whether the cell has i sites.
+// span = 2: the cell and the next one in x together, over their common
+// neighbourhood (the i sites of the second cell move into the first cell's
+// FP32 frame). Returns whether the cells have i sites.
 template <int TI, typename R, bool COUNT, bool MOLSITE, bool MULTI>
 __device__ __forceinline__ bool pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE, MULTI> &T,
-                                          int cell, int lane, unsigned long long &ntest,
-                                          double (&wmac)[3]) {
+                                          int cell, int span, int lane,
+                                          unsigned long long &ntest, double (&wmac)[3]) {
     bool empty;
     {
         const int rep = cell_replica(a.R, cell);
         const Geom &g = a.R.geom[rep];
         const int cbase = a.R.cell_off[rep], local = cell - cbase;
         const SiteArrays &S = a.s[TI];
-        const int i_begin = S.cstart[cell], i_end = S.cstart[cell + 1];
+        const int i_begin = S.cstart[cell], i_end = S.cstart[cell + span];
+        const int i_mid = span > 1 ? S.cstart[cell + 1] : i_end;  // the second cell's first site
+        const int nxs = span + 2;
         empty = i_begin == i_end;
         const int cx = local % g.n[0], cy = (local / g.n[0]) % g.n[1],
                   cz = local / (g.n[0] * g.n[1]);
@@ -578,6 +587,13 @@ __device__ __forceinline__ bool pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             ISite<R> I;
             I.pos = SiteView<R>::pos(S, i);
             I.mol = MOLSITE ? I.pos : SiteView<R>::mol(S, i);
+            if constexpr (sizeof(R) == 4) {
+                if (i >= i_mid) {  // second cell: one cell edge further in x
+                    const float sx = (float)g.cl[0];
+                    I.pos.x += sx;
+                    I.mol.x += sx;
+                }
+            }
             I.idx = i;
             if constexpr (TI >= T_D) I.e = SiteView<R>::ori(S, i);
             if constexpr (TI == T_LJ) {
@@ -589,20 +605,20 @@ __device__ __forceinline__ bool pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             for (int k = 0; k < 3; ++k) acc.f[k] = acc.v[k] = acc.m[k] = 0.0;
             acc.u = acc.vir = acc.rf = 0.0;
             if constexpr (TI == T_LJ) {
-                partner_pass<T_LJ, T_LJ, R, MOLSITE, MULTI, COUNT>(a, g, cbase, I, valid, cx, cy, cz, ph, P,
+                partner_pass<T_LJ, T_LJ, R, MOLSITE, MULTI, COUNT>(a, g, cbase, I, valid, cx, cy, cz, nxs, ph, P,
                                                                    lane, T, acc, eps24, sig2,
                                                                    shift6, ntest);
             } else {
                 if (a.present[T_C])
-                    partner_pass<TI, T_C, R, MOLSITE, MULTI, COUNT>(a, g, cbase, I, valid, cx, cy, cz, ph, P,
+                    partner_pass<TI, T_C, R, MOLSITE, MULTI, COUNT>(a, g, cbase, I, valid, cx, cy, cz, nxs, ph, P,
                                                                     lane, T, acc, eps24, sig2,
                                                                     shift6, ntest);
                 if (a.present[T_D])
-                    partner_pass<TI, T_D, R, MOLSITE, MULTI, COUNT>(a, g, cbase, I, valid, cx, cy, cz, ph, P,
+                    partner_pass<TI, T_D, R, MOLSITE, MULTI, COUNT>(a, g, cbase, I, valid, cx, cy, cz, nxs, ph, P,
                                                                     lane, T, acc, eps24, sig2,
                                                                     shift6, ntest);
                 if (a.present[T_Q])
-                    partner_pass<TI, T_Q, R, MOLSITE, MULTI, COUNT>(a, g, cbase, I, valid, cx, cy, cz, ph, P,
+                    partner_pass<TI, T_Q, R, MOLSITE, MULTI, COUNT>(a, g, cbase, I, valid, cx, cy, cz, nxs, ph, P,
                                                                     lane, T, acc, eps24, sig2,
                                                                     shift6, ntest);
             }
@@ -657,9 +673,9 @@ __device__ __forceinline__ bool pair_cell(const PairArgs &a, Tile<TI, R, MOLSITE
             wmac[2] += acc.rf;
         }
     }
-    // a zero row (the sums of the work item's cells go to one row at the end)
+    // zero rows (the sums of the work item's cells go to one row at the end)
     if (lane == 0 && !a.accumulate)
-        for (int k = 0; k < 4; ++k) a.macro_part[4 * (size_t)cell + k] = 0.0;
+        for (int k = 0; k < 4 * span; ++k) a.macro_part[4 * (size_t)cell + k] = 0.0;
     return !empty;
 }
 
@@ -698,14 +714,30 @@ __global__ __launch_bounds__(64) __attribute__((
         const int c1 = min(c0 + a.cpb, a.ncell);
         double wmac[3] = {0.0, 0.0, 0.0};
         int last = -1, rep_prev = -1;
-        for (int cell = c0; cell < c1; ++cell) {
+        for (int cell = c0; cell < c1;) {
             const int rep = cell_replica(a.R, cell);
             if (rep != rep_prev && last >= 0) {  // rows never mix replicas
                 flush_macro<TI>(a, wmac, last, lane);
                 last = -1;
             }
             rep_prev = rep;
-            if (pair_cell<TI, R, COUNT, MOLSITE, MULTI>(a, T, cell, lane, ntest, wmac)) last = cell;
+            int span = 1;
+            if constexpr (TileCfg<TI, R, MOLSITE, MULTI>::JOINT) {
+                // pairs of cells in one x row of one replica, where the cell
+                // edge exceeds the cutoff: the 9 cells of the common
+                // neighbourhood that are not neighbours of one of the two are
+                // then at least a cell edge, more than rc, from its sites, so
+                // testing them adds no pair that ls1's traversal leaves out
+                const Geom &g = a.R.geom[rep];
+                const int local = cell - a.R.cell_off[rep];
+                const double rc2 = TI == T_LJ ? a.rclj2 : a.rc2;
+                if (cell + 1 < c1 && local % g.n[0] + 1 < g.n[0] &&
+                    g.cl[0] * g.cl[0] > rc2 * (1.0 + 1e-9))
+                    span = 2;
+            }
+            if (pair_cell<TI, R, COUNT, MOLSITE, MULTI>(a, T, cell, span, lane, ntest, wmac))
+                last = cell + span - 1;
+            cell += span;
         }
         if (last >= 0) flush_macro<TI>(a, wmac, last, lane);
     }
