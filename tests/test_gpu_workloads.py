@@ -133,6 +133,25 @@ def test_mixture_dense_cells():
     check_fp32(gpu_eval(s, PREC_FP32_FP64ACC), ref)
 
 
+def test_mixture_cell_edge_equal_to_cutoff():
+    """A box of exactly 6 rc: the cell edge equals the cutoff, so the
+    quadrupole kernel runs cell by cell instead of two cells per tile (the
+    common tile holds cells a cell edge away from one of the two, which are
+    only out of range when the edge exceeds rc); both forms against the
+    oracle as in the mixture test. The other mixture tests cover the two-cell
+    tiles (cell edges above rc) and a row of three cells."""
+    s = scenarios.mixture(n=3000)
+    scale = 6.0 * s.rc / s.box[0]
+    s.r = s.r * scale
+    s.box = np.full(3, 6.0 * s.rc)
+    say("case mixture box 6 rc N", s.n, "box", s.box[0])
+    o = oracle_for(s)
+    ref = o.forces(s.r, s.q, s.cid)
+    assert np.abs(ref["M"]).max() > 0
+    check_fp64(gpu_eval(s, PREC_FP64), ref)
+    check_fp32(gpu_eval(s, PREC_FP32_FP64ACC), ref)
+
+
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32_FP64ACC])
 def test_replica_batch_each_vs_oracle(prec):
     reps = scenarios.replicas(nrep=12, nmin=2000, nmax=4000)
