@@ -36,8 +36,8 @@ __global__ void k_scan_block(ScanArgs);
 __global__ void k_scan_add(ScanArgs);
 __global__ void k_scan_total(ScanArgs);
 __global__ void k_cell_site_start(int, const int *, ScanArgs);
-__global__ void k_build_sites(int, MolArrays, Reps, const int *, const Component *, int,
-                              SiteArrays, SiteArrays, SiteArrays, SiteArrays);
+__global__ void k_build_sites(int, MolArrays, Reps, const int *, const Component *, int, int,
+                              int, SiteArrays, SiteArrays, SiteArrays, SiteArrays);
 __global__ void k_calc_fm(int, MolArrays, const Component *, int, SiteArrays, SiteArrays,
                           SiteArrays, SiteArrays, double *, double *, double *, Reps,
                           const double *);
@@ -152,6 +152,7 @@ struct mdx_context {
         size_t seg;
     };
     std::vector<ScanLevel> scan_tmp;
+    int site_forms = 0;  // site record forms built for the current re-bin (1 FP64, 2 FP32)
     double *macro_part = nullptr, *lf_part = nullptr, *red = nullptr, *step_out = nullptr;
     int *step_idx = nullptr;
     int cap_steps = 0;
@@ -821,10 +822,14 @@ int launch_update(mdx_context *c) {
         hipLaunchKernelGGL(k_cell_site_start, dim3(nblk(c->ncell + 1), na), dim3(kBlock), 0,
                            c->stream, c->ncell, c->start, cs);
     }
+    // the site records of the current precision form only; another form is
+    // added on demand (ensure_sites)
     const int nc = c->model.ncomp;
+    const int forms = c->precision == MDX_PREC_FP32_FP64ACC ? 2 : 1;
     hipLaunchKernelGGL(k_build_sites, dim3(nblk(n)), dim3(kBlock), nc * sizeof(Component),
-                       c->stream, n, c->mol[c->cur], reps(c), c->mcell, c->d_comps, nc,
-                       c->sites[0], c->sites[1], c->sites[2], c->sites[3]);
+                       c->stream, n, c->mol[c->cur], reps(c), c->mcell, c->d_comps, nc, forms,
+                       1, c->sites[0], c->sites[1], c->sites[2], c->sites[3]);
+    c->site_forms = forms;
     c->binned = true;
     HIPCHK(c, hipGetLastError());
     return MDX_OK;
@@ -837,9 +842,26 @@ int ensure_binned(mdx_context *c) {
     return MDX_OK;
 }
 
+// The site records of form bit(s) need (1 FP64, 2 FP32) for the current
+// re-bin: the re-bin builds its precision's form; a traversal in the other
+// precision or the (FP64) adjoint adds the missing one from the same q.
+// One-centre systems read molecule arrays and the gather's FP32 cache.
+int ensure_sites(mdx_context *c, int need) {
+    if (c->point_lj || (c->site_forms & need) == need) return MDX_OK;
+    const int nc = c->model.ncomp, n = c->n;
+    hipLaunchKernelGGL(k_build_sites, dim3(nblk(n)), dim3(kBlock), nc * sizeof(Component),
+                       c->stream, n, c->mol[c->cur], reps(c), c->mcell, c->d_comps, nc,
+                       need & ~c->site_forms, 0, c->sites[0], c->sites[1], c->sites[2],
+                       c->sites[3]);
+    c->site_forms |= need;
+    HIPCHK(c, hipGetLastError());
+    return MDX_OK;
+}
+
 // kick != 0 (reduced-memory mode): the pair kernel adds kick * F to the
 // velocities instead of storing F and Vi.
 int launch_traverse(mdx_context *c, double kick = 0.0) {
+    TRY(ensure_sites(c, c->precision == MDX_PREC_FP32_FP64ACC ? 2 : 1));
     PairArgs a;
     memset(&a, 0, sizeof(a));
     a.R = reps(c);
@@ -1974,6 +1996,7 @@ int adjoint_run(mdx_context *c, const double *cot, double *dU_deps, double *dU_d
                 const AdjBufs &b) {
     HIPCHK(c, hipSetDevice(c->device));
     TRY(ensure_binned(c));
+    TRY(ensure_sites(c, 1));
     const int nl = c->model.nlj;
     const int nn = c->n, ncomp = c->model.ncomp;
     const int nsl = c->nsites[T_LJ], nsc = c->nsites[T_C];

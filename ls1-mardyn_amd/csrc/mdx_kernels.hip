@@ -642,18 +642,28 @@ __device__ __forceinline__ const Component *stage_comps(const Component *__restr
 // reference mutates the state here, Quaternion.h:40-42), rotate the body-frame
 // sites, record molecule position and parameters; positions also relative to
 // the molecule's cell corner in FP32.
+// forms: bit 0 the FP64 site records (positions, orientations, parameters)
+// the FP64 pair kernels and the adjoint read, bit 1 the FP32 ones of the
+// FP32-pair form. first: the re-bin's build, which also normalises q and
+// writes what both forms read (molecule index and FP64 molecule position of
+// each site, LJ ids, charge head counts); a later call with first = 0 adds
+// a form the re-bin did not build, from the same (normalised) q.
 __global__ __launch_bounds__(256) void k_build_sites(int n, MolArrays s, Reps R,
                                                      const int *__restrict__ mcell,
                                                      const Component *__restrict__ gcomps,
-                                                     int ncomp, SiteArrays s0, SiteArrays s1,
+                                                     int ncomp, int forms, int first,
+                                                     SiteArrays s0, SiteArrays s1,
                                                      SiteArrays s2, SiteArrays s3) {
     const Component *comps = stage_comps(gcomps, ncomp);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double q[4] = {s.qw[i], s.qx[i], s.qy[i], s.qz[i]};
-    double qs = 1. / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-    for (int k = 0; k < 4; ++k) q[k] *= qs;
-    s.qw[i] = q[0]; s.qx[i] = q[1]; s.qy[i] = q[2]; s.qz[i] = q[3];
+    if (first) {
+        double qs = 1. / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        for (int k = 0; k < 4; ++k) q[k] *= qs;
+        s.qw[i] = q[0]; s.qx[i] = q[1]; s.qy[i] = q[2]; s.qz[i] = q[3];
+    }
+    const bool f64 = forms & 1, f32 = forms & 2;
     const Component &c = comps[s.cid[i]];
     double r[3] = {s.rx[i], s.ry[i], s.rz[i]};
     const int rep = mol_replica(R, i);
@@ -674,25 +684,28 @@ __global__ __launch_bounds__(256) void k_build_sites(int n, MolArrays s, Reps R,
             double d[3], pos[3];
             qrot(q, c.body[t][k], d);
             for (int a = 0; a < 3; ++a) pos[a] = d[a] + r[a];
-            A.mol[j] = i;
-            A.x[j] = pos[0]; A.y[j] = pos[1]; A.z[j] = pos[2];
-            A.mx[j] = r[0]; A.my[j] = r[1]; A.mz[j] = r[2];
-            A.fx[j] = (float)(pos[0] - corner[0]);
-            A.fy[j] = (float)(pos[1] - corner[1]);
-            A.fz[j] = (float)(pos[2] - corner[2]);
-            A.fmx[j] = mrel[0]; A.fmy[j] = mrel[1]; A.fmz[j] = mrel[2];
-            if (t == T_LJ) {
-                A.ljid[j] = c.lj_base + k;
-            } else {
-                A.p[j] = c.param[t][k];
-                A.fp[j] = (float)c.param[t][k];
+            if (first) {
+                A.mol[j] = i;
+                A.mx[j] = r[0]; A.my[j] = r[1]; A.mz[j] = r[2];
+                if (t == T_LJ) A.ljid[j] = c.lj_base + k;
+                if (t == T_C) A.nhead[j] = k == 0 ? c.n[T_C] : 0;
             }
-            if (t == T_C) A.nhead[j] = k == 0 ? c.n[T_C] : 0;
+            if (f64) {
+                A.x[j] = pos[0]; A.y[j] = pos[1]; A.z[j] = pos[2];
+                if (t != T_LJ) A.p[j] = c.param[t][k];
+            }
+            if (f32) {
+                A.fx[j] = (float)(pos[0] - corner[0]);
+                A.fy[j] = (float)(pos[1] - corner[1]);
+                A.fz[j] = (float)(pos[2] - corner[2]);
+                A.fmx[j] = mrel[0]; A.fmy[j] = mrel[1]; A.fmz[j] = mrel[2];
+                if (t != T_LJ) A.fp[j] = (float)c.param[t][k];
+            }
             if (t >= T_D) {
                 double e[3];
                 qrot(q, c.e[t][k], e);
-                A.ex[j] = e[0]; A.ey[j] = e[1]; A.ez[j] = e[2];
-                A.fex[j] = (float)e[0]; A.fey[j] = (float)e[1]; A.fez[j] = (float)e[2];
+                if (f64) { A.ex[j] = e[0]; A.ey[j] = e[1]; A.ez[j] = e[2]; }
+                if (f32) { A.fex[j] = (float)e[0]; A.fey[j] = (float)e[1]; A.fez[j] = (float)e[2]; }
             }
         }
     }

@@ -1,6 +1,9 @@
 """Calls that read the cell lists right after an upload (no
 mdx_update_container first) bin the state themselves: traversal, autotuner
 and cell ids give what the documented order (upload, update, traverse)
+gives. A re-bin builds the site records of the context's precision only;
+a traversal in the other precision, or the FP64 adjoint, on the same binned
+state adds the missing records and gives what a context of that precision
 gives."""
 import sys
 
@@ -10,6 +13,7 @@ import pytest
 import scenarios
 from ls1amd import PREC_FP32_FP64ACC, PREC_FP64, Context
 from oracle_lib import Oracle
+from test_oracle_adjoint import small_charged
 
 pytestmark = pytest.mark.gpu
 
@@ -54,3 +58,41 @@ def test_autotune_and_cell_ids_right_after_upload():
         out = c.download()
     ref = o.forces(s.r, s.q, s.cid)
     assert np.abs(out["F"] - ref["F"]).max() < 1e-12 * np.abs(ref["F"]).max()
+
+
+@pytest.mark.parametrize("first", [PREC_FP64, PREC_FP32_FP64ACC])
+def test_precision_switch_without_rebin(first):
+    say("api order precision switch, first", first)
+    s = scenarios.small_mixture(seed=93)
+    ref = {}
+    for p in (PREC_FP64, PREC_FP32_FP64ACC):
+        with Context(0) as c:
+            c.setup_system(s, periodic=True, precision=p)
+            c.update_container()
+            ref[p] = (c.traverse(), c.download())
+    other = PREC_FP32_FP64ACC if first == PREC_FP64 else PREC_FP64
+    with Context(0) as c:
+        c.setup_system(s, periodic=True, precision=first)
+        c.update_container()
+        for p in (first, other, first, other):
+            c.set_precision(p)
+            m = c.traverse()
+            out = c.download()
+            np.testing.assert_array_equal(m, ref[p][0])
+            for key in ("F", "M", "Vi"):
+                assert np.array_equal(out[key], ref[p][1][key]), (p, key)
+
+
+def test_adjoint_in_fp32_context():
+    say("api order adjoint after an FP32 re-bin")
+    s = small_charged()
+    cot = np.random.default_rng(5).normal(size=(s.n, 3))
+    got = {}
+    for p in (PREC_FP64, PREC_FP32_FP64ACC):
+        with Context(0) as c:
+            c.setup_system(s, periodic=True, precision=p)
+            c.update_container()
+            c.traverse()
+            got[p] = c.adjoint(cot)
+    for k, v in got[PREC_FP64].items():
+        assert np.array_equal(got[PREC_FP32_FP64ACC][k], v), k
