@@ -770,6 +770,11 @@ SiteArrays site_view(const mdx_context *c, int t) {
     return s;
 }
 
+// The site record forms (1 FP64, 2 FP32) a re-bin builds: its precision's.
+int rebin_forms(const mdx_context *c) {
+    return c->precision == MDX_PREC_FP32_FP64ACC ? 2 : 1;
+}
+
 // Re-bin (LinkedCells::update + exchangeMolecules' periodic wrap, which
 // leapfrog_pre already applied) and SoA cache build (setupSoACache).
 int launch_update(mdx_context *c) {
@@ -825,7 +830,7 @@ int launch_update(mdx_context *c) {
     // the site records of the current precision form only; another form is
     // added on demand (ensure_sites)
     const int nc = c->model.ncomp;
-    const int forms = c->precision == MDX_PREC_FP32_FP64ACC ? 2 : 1;
+    const int forms = rebin_forms(c);
     hipLaunchKernelGGL(k_build_sites, dim3(nblk(n)), dim3(kBlock), nc * sizeof(Component),
                        c->stream, n, c->mol[c->cur], reps(c), c->mcell, c->d_comps, nc, forms,
                        1, c->sites[0], c->sites[1], c->sites[2], c->sites[3]);
@@ -861,7 +866,7 @@ int ensure_sites(mdx_context *c, int need) {
 // kick != 0 (reduced-memory mode): the pair kernel adds kick * F to the
 // velocities instead of storing F and Vi.
 int launch_traverse(mdx_context *c, double kick = 0.0) {
-    TRY(ensure_sites(c, c->precision == MDX_PREC_FP32_FP64ACC ? 2 : 1));
+    TRY(ensure_sites(c, rebin_forms(c)));
     PairArgs a;
     memset(&a, 0, sizeof(a));
     a.R = reps(c);
@@ -1812,6 +1817,8 @@ int mdx_step(mdx_context *c, int nsteps, double dt, double *per_step, int use_gr
             }
             HIPCHK(c, hipGraphLaunch(gx, c->stream));
             c->cur = 1 - parity;
+            // the replayed re-bin rebuilt only its precision's site records
+            if (!c->point_lj) c->site_forms = rebin_forms(c);
         }
     } else {
         for (int s = 0; s < nsteps; ++s) TRY(launch_step(c, dt, kind_of(s)));

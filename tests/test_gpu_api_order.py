@@ -96,3 +96,35 @@ def test_adjoint_in_fp32_context():
             got[p] = c.adjoint(cot)
     for k, v in got[PREC_FP64].items():
         assert np.array_equal(got[PREC_FP32_FP64ACC][k], v), k
+
+
+def test_adjoint_after_graph_steps_in_fp32_context():
+    """Graph-launched steps re-bin on the device; the FP64 records an earlier
+    adjoint added belong to the old binning and must be rebuilt for the
+    next adjoint."""
+    say("api order adjoint after graph-launched FP32 steps")
+    s = small_charged()
+    rng = np.random.default_rng(6)
+    s.v = rng.normal(scale=0.5, size=(s.n, 3))
+    cot = rng.normal(size=(s.n, 3))
+    with Context(0) as c:
+        c.setup_system(s, periodic=True, precision=PREC_FP32_FP64ACC)
+        c.update_container()
+        c.traverse()
+        c.step(2, 0.002, use_graph=True)
+        c.adjoint(cot)
+        c.step(3, 0.002, use_graph=True)
+        got = c.adjoint(cot)
+        st = c.download()
+    s.r, s.v, s.q, s.L = st["r"], st["v"], st["q"], st["L"]
+    with Context(0) as c:
+        c.setup_system(s, periodic=True, precision=PREC_FP64)
+        c.update_container()
+        c.traverse()
+        ref = c.adjoint(cot)
+    # the fresh context normalises the downloaded q once more (last-bit
+    # changes); stale records of the earlier binning differ by the steps'
+    # displacements
+    for k, v in ref.items():
+        scale = max(np.abs(v).max(), 1e-300)
+        np.testing.assert_allclose(got[k], v, rtol=0, atol=1e-9 * scale, err_msg=k)
