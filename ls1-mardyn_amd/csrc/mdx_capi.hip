@@ -879,6 +879,9 @@ int launch_traverse(mdx_context *c, double kick = 0.0) {
     }
     a.vscale = c->point_lj ? 0.5 : 1.0;
     a.thermo = (c->lrc && c->point_lj) ? c->d_thermo : nullptr;
+    a.mrx = c->mol[c->cur].rx;
+    a.mry = c->mol[c->cur].ry;
+    a.mrz = c->mol[c->cur].rz;
     if (kick != 0.0) {
         const MolArrays &m = c->mol[c->cur];
         a.kvx = m.vx;

@@ -150,6 +150,7 @@ struct PairArgs {
                         // arrays are the molecule arrays (calcFM's Vi = 0.5 sum V_site)
     const double *thermo;  // [nrep][kThermoW] when the written virial also takes the
                            // long-range correction's per-molecule addition, else null
+    const double *mrx, *mry, *mrz;  // molecule positions (the FP32 form's FP64 re-check)
     double *kvx, *kvy, *kvz;  // reduced-memory mode: the molecule velocities that take
     double kick;              // v += kick * F instead of storing F and Vi (else null)
     unsigned long long *counters;  // [8] or null

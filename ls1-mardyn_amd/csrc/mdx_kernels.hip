@@ -642,12 +642,12 @@ __device__ __forceinline__ const Component *stage_comps(const Component *__restr
 // reference mutates the state here, Quaternion.h:40-42), rotate the body-frame
 // sites, record molecule position and parameters; positions also relative to
 // the molecule's cell corner in FP32.
-// forms: bit 0 the FP64 site records (positions, orientations, parameters)
-// the FP64 pair kernels and the adjoint read, bit 1 the FP32 ones of the
-// FP32-pair form. first: the re-bin's build, which also normalises q and
-// writes what both forms read (molecule index and FP64 molecule position of
-// each site, LJ ids, charge head counts); a later call with first = 0 adds
-// a form the re-bin did not build, from the same (normalised) q.
+// forms: bit 0 the FP64 site records (positions, molecule positions,
+// orientations, parameters) the FP64 pair kernels and the adjoint read, bit 1
+// the FP32 ones of the FP32-pair form. first: the re-bin's build, which also
+// normalises q and writes what both forms read (molecule index of each site,
+// LJ ids, charge head counts); a later call with first = 0 adds a form the
+// re-bin did not build, from the same (normalised) q.
 __global__ __launch_bounds__(256) void k_build_sites(int n, MolArrays s, Reps R,
                                                      const int *__restrict__ mcell,
                                                      const Component *__restrict__ gcomps,
@@ -686,12 +686,12 @@ __global__ __launch_bounds__(256) void k_build_sites(int n, MolArrays s, Reps R,
             for (int a = 0; a < 3; ++a) pos[a] = d[a] + r[a];
             if (first) {
                 A.mol[j] = i;
-                A.mx[j] = r[0]; A.my[j] = r[1]; A.mz[j] = r[2];
                 if (t == T_LJ) A.ljid[j] = c.lj_base + k;
                 if (t == T_C) A.nhead[j] = k == 0 ? c.n[T_C] : 0;
             }
             if (f64) {
                 A.x[j] = pos[0]; A.y[j] = pos[1]; A.z[j] = pos[2];
+                A.mx[j] = r[0]; A.my[j] = r[1]; A.mz[j] = r[2];
                 if (t != T_LJ) A.p[j] = c.param[t][k];
             }
             if (f32) {

@@ -349,10 +349,12 @@ __device__ __forceinline__ void partner_pass(const PairArgs &a, const Geom &g, i
             const int idx = base + s;
             const int k = tile_k(T.pre, idx, nb);
             const int j = T.j0[k] + (idx - T.pre[k]);
-            const SiteArrays &S = a.s[TI];
-            const double ex = S.mx[I.idx] - (J.mx[j] + T.shift[k][0]);
-            const double ey = S.my[I.idx] - (J.my[j] + T.shift[k][1]);
-            const double ez = S.mz[I.idx] - (J.mz[j] + T.shift[k][2]);
+            // molecule positions through the sites' molecule indices (the
+            // FP32 form builds no FP64 per-site copies)
+            const int mi = a.s[TI].mol[I.idx], mj = J.mol[j];
+            const double ex = a.mrx[mi] - (a.mrx[mj] + T.shift[k][0]);
+            const double ey = a.mry[mi] - (a.mry[mj] + T.shift[k][1]);
+            const double ez = a.mrz[mi] - (a.mrz[mj] + T.shift[k][2]);
             const double r2d = ex * ex + ey * ey + ez * ez;
             return r2d < rc2 && r2d != 0.0;
         };
